@@ -1,0 +1,180 @@
+"""Benchmark: D3Q19 Smagorinsky LES MLUPS and HBM-roofline fraction on one MI355X.
+
+Workload (default): the plain D3Q19 LES at 256x128x128 with FP16 deviation storage, Bouzidi walls,
+moving belt at u = 0.08 with rolling wheels, Cs = 0.1, around the procedural sedan (rounded box
+85x22x32 cells, corner radius 6, four wheels r = 6, width 5, ride height 3). One step = one fused
+pull-stream + collide pass over the whole grid (bulk kernel + boundary kernel).
+
+Prints ONE JSON line. `value` is MLUPS over the timed steps with the populations resident in HBM.
+`roofline` prices the dominant kernel (the bulk stream-collide kernel) at 2*Q*s algorithmic bytes
+per cell it updates, divided by its average launch time measured with HIP events on the solver's
+stream, against the 8 TB/s HBM peak. `cpu_baseline` times the CPU restatement on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "D3Q19 LES MLUPS + % HBM roofline; learned vs plain time-to-C_D at 2% error"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params(u=0.08, precision="f16"):
+    # config: Cs 0.1, u 0.08, moving belt, interp; nu = 6e-4 (the app's 3D floor) and the app's
+    # sponge / wall defaults (app.ts:227-231)
+    return dict(u=u, nu=6e-4, cs=0.1, ground="moving", sides="freeslip", spongeNu=0.04, spongeStart=0.84,
+                spongeIn=0.03, tauWall=0.53, interp=True, collision="regularized", emaAlpha=0.02, precision=precision)
+
+
+def make_sedan_solver(dims, precision="f16"):
+    from windtunnel_vis_amd import SolverGPU
+    from windtunnel_vis_amd import geometry as G
+    s = SolverGPU(dims, workload_params(precision=precision))
+    body = G.sedan(dims)
+    s.setWheels(body.wheels)
+    info = s.voxelizeSdf(body.prims())
+    return s, body, info
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per bulk launch from a committed PMC summary (FETCH_SIZE + WRITE_SIZE, scaled)."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def run_gpu(dims, steps, warmup, precision="f16"):
+    s, body, info = make_sedan_solver(dims, precision)
+    q = s.lattice.q
+    sbytes = 2 if s.precision != "f32" else 4
+    n = s.n
+    s.step(warmup)
+    s.sync()
+    s.profileEnable(True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    t1 = time.perf_counter()
+    bulk_ms, launches, bnd_ms = s.profileRead()
+    s.profileEnable(False)
+    st = s.stepAndSample(0)
+    bulk_cells, bnd_cells = s.workSplit()
+    wall = t1 - t0
+    mlups = n * steps / wall / 1e6
+    bulk_avg_s = bulk_ms / 1e3 / max(launches, 1)
+    bulk_bytes = bulk_cells * 2 * q * sbytes
+    achieved = bulk_bytes / bulk_avg_s / 1e9
+    return dict(solver=s, body=body, info=info, mlups=mlups, wall=wall, bulk_avg_ms=bulk_avg_s * 1e3,
+                bnd_avg_ms=bnd_ms / max(launches, 1), bulk_cells=bulk_cells, bnd_cells=bnd_cells, bulk_bytes=bulk_bytes,
+                achieved_gbs=achieved, unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes)
+
+
+def physics(s, body, info, u, total=5000, chunk=250):
+    """C_D / C_L averaged over the second half of a `total`-step run (harness.ts:172-183)."""
+    from windtunnel_vis_amd.analysis import coefficients
+    s.reset()
+    acc_d, acc_l, k = 0.0, 0.0, 0
+    unstable = False
+    for st_ in range(0, total, chunk):
+        r = s.stepAndSample(chunk)
+        c = coefficients(r.fx, r.fy, r.fz, u, info["frontal"])
+        if st_ > total / 2:
+            acc_d += c["cd"]
+            acc_l += c["cl"]
+            k += 1
+        if r.unstable:
+            unstable = True
+            break
+    return dict(cd=acc_d / max(k, 1), cl=acc_l / max(k, 1), unstable=unstable, frontal=info["frontal"])
+
+
+def cpu_baseline(dims, sample_steps, threads):
+    """The CPU restatement (FP32 arithmetic path, OpenMP) on the same domain for a few steps."""
+    from oracle.oracle import OracleSolver
+    from windtunnel_vis_amd import geometry as G
+    p = workload_params(precision="f32")
+    o = OracleSolver(dims, p, storage="f16", fp32=True, threads=threads)
+    body = G.sedan(dims)
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+    d = body.sdf(np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1))
+    flags = (d < 0).astype(np.uint32)
+    o.setWheels(body.wheels)
+    o.uploadFlags(flags)
+    o.step(1)  # init + first touch
+    t0 = time.perf_counter()
+    o.step(sample_steps)
+    dt = time.perf_counter() - t0
+    n = nx * ny * nz
+    return dict(value=n * sample_steps / dt / 1e6, seconds=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--dims", default="256,128,128")
+    ap.add_argument("--precision", default="f16")
+    ap.add_argument("--no-physics", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=0, help="CPU-baseline sample steps (0 = auto)")
+    ap.add_argument("--north-star", action="store_true", help="also time 512x256x256")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    dims = tuple(int(v) for v in args.dims.split(","))
+    r = run_gpu(dims, args.steps, args.warmup, args.precision)
+    out = {
+        "metric": METRIC, "value": round(r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "D3Q19 Smagorinsky LES %dx%dx%d, %s deviation storage, procedural sedan SDF, "
+                               "Bouzidi walls, moving belt + rolling wheels, u=0.08, Cs=0.1" % (dims + (args.precision.upper(),)),
+                   "grid": list(dims), "storage": args.precision, "collision": "regularized", "closure": "smagorinsky"},
+    }
+    frac = r["achieved_gbs"] / HBM_PEAK_GBS
+    out["roofline"] = {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(frac, 4),
+                       "traffic": traffic_from_profiles("pmc_bulk_d3q19_f16_%dx%dx%d.txt" % dims),
+                       "kernel": "k_bulk<D3Q19,SF16>", "bytes_per_launch": r["bulk_bytes"],
+                       "avg_launch_ms": round(r["bulk_avg_ms"], 5), "cells_per_launch": r["bulk_cells"]}
+    out["step_roofline_frac"] = round(r["mlups"] * 1e6 * 2 * r["q"] * r["sbytes"] / (HBM_PEAK_GBS * 1e9), 4)
+    out["boundary_kernel_ms"] = round(r["bnd_avg_ms"], 5)
+    out["unstable"] = bool(r["unstable"])
+    if not args.no_physics:
+        ph = physics(r["solver"], r["body"], r["info"], 0.08)
+        out["physics"] = {"cd": round(ph["cd"], 4), "cl": round(ph["cl"], 4), "frontal_cells": ph["frontal"],
+                          "unstable": ph["unstable"], "steps": 5000}
+    r["solver"].destroy()
+    if args.north_star:
+        big = (512, 256, 256)
+        rb = run_gpu(big, max(args.steps // 4, 200), args.warmup, args.precision)
+        out["north_star_512x256x256"] = {"mlups": round(rb["mlups"], 1),
+                                          "step_roofline_frac": round(rb["mlups"] * 1e6 * 2 * rb["q"] * rb["sbytes"] / (HBM_PEAK_GBS * 1e9), 4),
+                                          "bulk_frac": round(rb["achieved_gbs"] / HBM_PEAK_GBS, 4)}
+        rb["solver"].destroy()
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        steps = args.cpu_steps or 20
+        cb = cpu_baseline(dims, steps, threads)
+        out["cpu_baseline"] = {"value": round(cb["value"], 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+                               "sample": "%d steps of the same %dx%dx%d sedan domain (FP32 CPU restatement, FP16 store "
+                                         "rounding, OpenMP), %.1f s" % ((steps,) + dims + (cb["seconds"],))}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,222 @@
+/*
+ * wt.h — C ABI of the MI355X lattice-Boltzmann wind-tunnel library (libwt.so).
+ *
+ * This is the drop-in boundary for the reference's solver surface: every entry point replaces one
+ * member of `SolverGPU` (src/solver/SolverGPU.ts), of the `LbmSolver` contract
+ * (src/backend/types.ts:7-18) or of the voxelizer host (src/voxelize/VoxelizerGPU.ts), cited
+ * per function below. Hosts bind it through N-API (Node), ctypes (Python) or plain C.
+ *
+ * Conventions
+ *  - Every call returns an int status: WT_OK (0) or a negative WT_E_* code; wt_last_error() gives a
+ *    thread-local message for the last failing call on this thread.
+ *  - The caller owns host arrays; the library owns all device memory.
+ *  - One HIP stream per solver handle. Handles are independent; one handle is not re-entrant.
+ *  - wt_step is asynchronous (it enqueues work and returns); wt_step_and_sample, the read calls and
+ *    the upload calls synchronise the handle's stream.
+ *  - Populations are exchanged with the host unpacked, as full values f_i (not deviations), in the
+ *    direction-major SoA order f[i * n + cell], cell = x + nx * (y + ny * z).
+ *  - Physics failure (NaN, over-speed) is reported through wt_stats.unstable, not as an error,
+ *    as in SolverGPU.ts:277.
+ */
+#ifndef WT_H
+#define WT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WT_OK 0
+#define WT_E_ARG -1      /* invalid argument (sizes, enum values, null pointers) */
+#define WT_E_HIP -2      /* a HIP runtime call failed (message has the HIP error string) */
+#define WT_E_NOMEM -3    /* device allocation failed */
+#define WT_E_STATE -4    /* call not valid in the handle's current state */
+#define WT_E_NODEV -5    /* no usable GPU */
+
+typedef struct wt_solver wt_solver;
+
+/* GridDims, src/solver/types.ts:56-60. nz == 1 selects D2Q9, nz > 1 D3Q19 (SolverGPU.ts:63-64). */
+typedef struct {
+  int32_t nx, ny, nz;
+} wt_dims;
+
+enum { WT_GROUND_FREESLIP = 0, WT_GROUND_NOSLIP = 1, WT_GROUND_MOVING = 2 };
+enum { WT_SIDES_FREESLIP = 0, WT_SIDES_PERIODIC = 1 };
+/* Population storage. F16/BF16 store deviations f_i - w_i; like SolverGPU.ts:69 a half-width format
+ * needs an even nx and otherwise falls back to F32 (wt_info reports the format in use). */
+enum { WT_PREC_F32 = 0, WT_PREC_F16 = 1, WT_PREC_BF16 = 2 };
+enum { WT_COLL_REGULARIZED = 0, WT_COLL_BGK = 1 };
+/* Sub-grid closure: Smagorinsky (wgslLbm.ts:214) or the learned per-cell MLP eddy viscosity. */
+enum { WT_CLOSURE_SMAGORINSKY = 0, WT_CLOSURE_MLP = 1 };
+
+/* SolverParams, src/solver/types.ts:3-29 (same fields and meaning; optional fields take the
+ * reference defaults when the caller passes them: sponge_in 0, tau_wall 0.5, interp 1,
+ * collision regularized). */
+typedef struct {
+  double u;            /* lattice inlet velocity */
+  double nu;           /* molecular lattice viscosity; tau0 = 3 nu + 1/2 */
+  double cs;           /* Smagorinsky constant */
+  int32_t ground;      /* WT_GROUND_* */
+  int32_t sides;       /* WT_SIDES_* (z boundaries in 3D) */
+  double sponge_nu;    /* extra viscosity in the outlet sponge */
+  double sponge_start; /* fraction of nx where the outlet sponge starts */
+  double sponge_in;    /* width (fraction of nx) of the inlet absorbing layer */
+  double tau_wall;     /* minimum relaxation time on floor-adjacent cells */
+  int32_t precision;   /* WT_PREC_*, fixed at creation */
+  int32_t interp;      /* 1: Bouzidi interpolated bounce-back, 0: halfway */
+  int32_t collision;   /* WT_COLL_* */
+  int32_t closure;     /* WT_CLOSURE_* */
+  double ema_alpha;    /* EMA weight of the time averages at each sample */
+} wt_params;
+
+/* Field mask for wt_set_params (Partial<SolverParams>, SolverGPU.ts:155-158). */
+#define WT_P_U (1u << 0)
+#define WT_P_NU (1u << 1)
+#define WT_P_CS (1u << 2)
+#define WT_P_GROUND (1u << 3)
+#define WT_P_SIDES (1u << 4)
+#define WT_P_SPONGE_NU (1u << 5)
+#define WT_P_SPONGE_START (1u << 6)
+#define WT_P_SPONGE_IN (1u << 7)
+#define WT_P_TAU_WALL (1u << 8)
+#define WT_P_INTERP (1u << 9)
+#define WT_P_COLLISION (1u << 10)
+#define WT_P_CLOSURE (1u << 11)
+#define WT_P_EMA_ALPHA (1u << 12)
+#define WT_P_ALL 0x1fffu
+
+/* SolverStats, src/solver/types.ts:31-42, decoded as SolverGPU.ts:302-308, in FP64. */
+typedef struct {
+  double fx, fy, fz;   /* momentum-exchange force on the body (lattice units) */
+  double max_u;        /* max |u| over fluid cells */
+  double rho_ref;      /* mean density on the reference plane x = min(4, nx-1) */
+  int32_t unstable;    /* NaN flag, non-finite fx or max u^2 > 0.25 */
+  int32_t pad;
+  int64_t step;        /* solver step at which the sample was taken */
+} wt_stats;
+
+/* Read-only facts about a handle. */
+typedef struct {
+  wt_dims dims;
+  int64_t n;           /* cells */
+  int32_t q;           /* 9 or 19 */
+  int32_t precision;   /* storage actually used (WT_PREC_*) */
+  int64_t step;        /* stepCount */
+  int64_t boundary_pairs; /* x-pairs of cells routed to the boundary kernel */
+  int64_t device_bytes;   /* device memory held by the handle */
+} wt_info;
+
+/* VoxelInfo, src/voxelize/VoxelizerGPU.ts (frontal cells, solid cells, bbox min / max+1). */
+typedef struct {
+  int64_t frontal;
+  int64_t solid_cells;
+  int32_t min[3];
+  int32_t max[3];
+} wt_voxel_info;
+
+/* ---- lifecycle -------------------------------------------------------------------------- */
+
+/* new SolverGPU(device, dims, params), SolverGPU.ts:61-144: allocates everything on the current
+ * HIP device, uploads all-fluid flags and unknown distances, and leaves an init pending
+ * (reset(), :190-194). */
+int wt_create(const wt_dims* dims, const wt_params* params, wt_solver** out);
+/* destroy(), SolverGPU.ts:432-437. Accepts NULL. */
+int wt_destroy(wt_solver* s);
+/* static bytesFor(dims), SolverGPU.ts:439-443 (the reference's FP32 estimate, in bytes). */
+int64_t wt_bytes_for(const wt_dims* dims);
+int wt_info_get(const wt_solver* s, wt_info* out);
+/* Thread-local message of the last failing call ("" if none). */
+const char* wt_last_error(void);
+
+/* ---- parameters / reset ----------------------------------------------------------------- */
+
+/* setParams(Partial<SolverParams>), SolverGPU.ts:155-187: copies the fields selected by mask. */
+int wt_set_params(wt_solver* s, const wt_params* p, uint32_t mask);
+int wt_get_params(const wt_solver* s, wt_params* out);
+/* tau0 getter, SolverGPU.ts:151-153. */
+double wt_tau0(const wt_solver* s);
+/* reset(), SolverGPU.ts:190-194: re-init both buffers at the next step, stepCount = 0, clear stats. */
+int wt_reset(wt_solver* s);
+/* clearStats(), SolverGPU.ts:196-201: meanB = 0, meanA = (0,0,0,1). */
+int wt_clear_stats(wt_solver* s);
+
+/* ---- geometry --------------------------------------------------------------------------- */
+
+/* uploadFlags(Uint32Array), SolverGPU.ts:342-344: 0 fluid, 1 solid, 1 | (k+1) << 8 wheel k. */
+int wt_upload_flags(wt_solver* s, const uint32_t* flags);
+/* readFlags(), SolverGPU.ts:347-356. */
+int wt_read_flags(wt_solver* s, uint32_t* out);
+/* uploadSdf(Float32Array), SolverGPU.ts:312-322: wall distance in cells, < 0 (or NaN) = unknown. */
+int wt_upload_sdf(wt_solver* s, const float* dist);
+/* setWheels(list), SolverGPU.ts:325-335: 8 floats per wheel (cx, cy, cz, ax, ay, az, r, hw),
+ * grid space, at most 16 (extra wheels are ignored, as in the reference). */
+int wt_set_wheels(wt_solver* s, const float* wheels, int32_t count);
+
+/* Analytic-SDF body for the voxelizer: a union of primitives in grid space. */
+enum { WT_PRIM_ROUNDED_BOX = 1, WT_PRIM_CYLINDER = 2, WT_PRIM_SPHERE = 3, WT_PRIM_WEDGE = 4 };
+typedef struct {
+  int32_t kind;        /* WT_PRIM_* */
+  int32_t flags;       /* reserved, 0 */
+  float center[3];     /* grid-space centre */
+  float rot[9];        /* row-major rotation R; local = R^T (p - center) */
+  float half[3];       /* box / wedge: half extents; cylinder: (radius, half width, 0); sphere: (r,0,0) */
+  float radius;        /* box / wedge corner radius */
+  float param;         /* wedge: slope of the top face (height lost per unit length), else 0 */
+} wt_prim;
+
+/* Voxelize an analytic body into the solver (no reference counterpart; the configs' procedural
+ * bodies). Writes flags (solid iff the cell centre is inside; cells inside a wheel cylinder of the
+ * solver's wheel list are tagged as VoxelizerGPU's wheel pass does, wgslVoxel.ts:268-290) and
+ * distances |sdf| < 2.5 at cell centres (the roundFlags convention, validation.ts:35-53). */
+int wt_voxelize_sdf(wt_solver* s, const wt_prim* prims, int32_t count, wt_voxel_info* info);
+
+/* VoxelizerGPU.voxelize(solver, mesh, matrix, wheels), VoxelizerGPU.ts:48-151: triangle mesh
+ * (positions xyz per vertex, 3 indices per triangle), column-major 4x4 mesh->grid matrix M, and
+ * the wheel list (8 floats per wheel, as wt_set_wheels; may be NULL/0). */
+int wt_voxelize_mesh(wt_solver* s, const float* positions, int32_t n_vertices, const uint32_t* indices,
+                     int32_t n_triangles, const float* matrix16, const float* wheels, int32_t n_wheels,
+                     wt_voxel_info* info);
+
+/* ---- stepping --------------------------------------------------------------------------- */
+
+/* encodeSteps(enc, steps), SolverGPU.ts:204-222: runs a pending init, then `steps` fused
+ * pull-stream + collide steps. Asynchronous. */
+int wt_step(wt_solver* s, int32_t steps);
+/* stepAndSample(steps), SolverGPU.ts:292-309: steps, then macro + momentum exchange + reduction,
+ * then a synchronous read of the decoded statistics. steps may be 0. */
+int wt_step_and_sample(wt_solver* s, int32_t steps, wt_stats* out);
+/* The undecoded 8-value reduction of the last sample as the reference's result buffer holds it
+ * (Fx, Fy, Fz, max u^2, sum rho_ref, n_ref, nan count, solid count), wgslLbm.ts:491-521. */
+int wt_sample_raw(wt_solver* s, double* out8);
+/* Block until the handle's stream is idle. */
+int wt_sync(wt_solver* s);
+
+/* ---- readback / upload ------------------------------------------------------------------ */
+
+/* readPopulations(), SolverGPU.ts:359-369, but unpacked to full f_i (q * n floats). */
+int wt_read_populations(wt_solver* s, float* out);
+/* No reference counterpart: overwrite the current populations (full f_i, q * n floats) after any
+ * pending init; rounds to the storage format. Seeded initial conditions and checkpoints. */
+int wt_upload_populations(wt_solver* s, const float* f);
+/* Macro fields of the last sample: velocity texture (ux, uy, uz, rho per cell, SolverGPU velTex),
+ * EMA means meanA (4 per cell) and meanB (1 per cell). Any pointer may be NULL. */
+int wt_read_macro(wt_solver* s, float* vel4, float* mean_a4, float* mean_b);
+
+/* ---- profiling -------------------------------------------------------------------------- */
+
+/* When enabled, every step records HIP events around its bulk kernel on the handle's stream.
+ * wt_profile_read synchronises and returns the summed bulk-kernel time (ms), the number of bulk
+ * launches timed, and the summed boundary-kernel time; then clears the accumulators. */
+int wt_profile_enable(wt_solver* s, int32_t on);
+int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* boundary_ms);
+
+/* Number of cells the bulk kernel updates per step (interior fluid + solid pairs) and the number
+ * the boundary kernel updates. */
+int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cells);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WT_H */

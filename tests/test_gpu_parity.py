@@ -1,0 +1,117 @@
+"""GPU vs oracle parity on tiny grids after 100 steps (populations, statistics, macro fields).
+
+Tolerances (max relative population error |f_gpu - f_oracle| / |f_oracle|):
+  FP32 storage vs FP64 oracle: 1e-5   (north-star bound)
+  FP16 deviation storage vs FP64 oracle with FP16 store rounding: 2e-3   (north-star bound)
+  BF16 deviation storage vs FP64 oracle with BF16 store rounding: 1e-2   (8-bit mantissa)
+"""
+import numpy as np
+import pytest
+
+from tests.cases import SEED, lattice_for, make_case, perturbed_state
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": 1e-5, "f16": 2e-3, "bf16": 1e-2}
+ORACLE_STORAGE = {"f32": None, "f16": "f16", "bf16": "bf16"}
+CASES = ["d3_full", "d3_periodic_bgk", "d3_freeslip_halfway", "d3_odd_f32", "d2_cyl", "d2_moving_bgk"]
+
+
+def _setup(solver, dims, params, flags, sdf, wheels, f0):
+    solver.setWheels(wheels)
+    solver.uploadFlags(flags)
+    solver.uploadSdf(sdf)
+    solver.uploadPopulations(f0)
+
+
+def _pair(name, prec):
+    from oracle.oracle import OracleSolver
+    from windtunnel_vis_amd import SolverGPU
+    dims, params, flags, sdf, wheels = make_case(name)
+    lat = lattice_for(dims)
+    p = dict(params, precision=prec)
+    g = SolverGPU(dims, p)
+    if prec != "f32" and g.precision == "f32":
+        pytest.skip("odd nx falls back to FP32 storage")
+    o = OracleSolver(dims, p, storage=ORACLE_STORAGE[prec])
+    f0 = perturbed_state(lat, dims, params["u"], flags, seed=SEED).astype(np.float32)
+    _setup(g, dims, p, flags, sdf, wheels, f0)
+    _setup(o, dims, p, flags, sdf, wheels, f0.astype(np.float64))
+    return g, o, lat
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("name", CASES)
+def test_population_parity_100_steps(name, prec):
+    g, o, lat = _pair(name, prec)
+    # the upload itself must round-trip to the storage format
+    a0, b0 = g.readPopulations().astype(np.float64), o.readPopulations()
+    e0 = np.max(np.abs(a0 - b0) / np.abs(b0))
+    assert e0 < TOL[prec] / 10, "upload mismatch %g" % e0
+    g.step(100)
+    o.step(100)
+    a, b = g.readPopulations().astype(np.float64), o.readPopulations()
+    assert np.all(np.isfinite(a))
+    err = np.abs(a - b) / np.abs(b)
+    k = int(np.argmax(err))
+    print("%s %s: max rel err %.3e at dir %d cell %d (gpu %.9g oracle %.9g)" % (
+        name, prec, err.max(), k // g.n, k % g.n, a.reshape(-1)[k], b.reshape(-1)[k]))
+    assert err.max() < TOL[prec]
+    assert g.stepCount == o.stepCount == 100
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+@pytest.mark.parametrize("name", CASES)
+def test_stats_parity(name, prec):
+    g, o, lat = _pair(name, prec)
+    for chunk in (0, 40, 60):
+        sg = g.stepAndSample(chunk)
+        so = o.stepAndSample(chunk)
+        rg, ro = g.sampleRaw(), o.sampleRaw()
+        print(name, prec, chunk, rg, ro)
+        # force: FP32 per-link terms summed in FP64 on both sides
+        scale = max(1.0, np.abs(ro[:3]).max())
+        tol = 1e-4 if prec == "f32" else 5e-3
+        assert np.all(np.abs(rg[:3] - ro[:3]) <= tol * scale)
+        assert abs(rg[3] - ro[3]) <= (1e-5 if prec == "f32" else 2e-3) * max(ro[3], 1e-6) + 1e-9
+        assert np.allclose(rg[4], ro[4], rtol=1e-5 if prec == "f32" else 1e-3)
+        assert rg[5] == ro[5] and rg[6] == ro[6] and rg[7] == ro[7]
+        assert sg.unstable == so.unstable and sg.step == so.step
+    vg, mag, mbg = g.readMacro()
+    vo, mao, mbo = o.readMacro()
+    # velocity texture is half precision (rgba16float in the reference)
+    assert np.allclose(vg, vo, rtol=2e-3, atol=2e-4)
+    assert np.allclose(mag, mao, rtol=1e-3, atol=1e-5)
+    assert np.allclose(mbg, mbo, rtol=1e-3, atol=1e-7)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16", "bf16"])
+def test_init_equilibrium(prec):
+    """reset() then readPopulations: feq(1,(U,0,0)) on fluid, w on solids (wgslLbm.ts:296-341)."""
+    from oracle.oracle import OracleSolver
+    from windtunnel_vis_amd import SolverGPU
+    dims, params, flags, sdf, wheels = make_case("d3_full")
+    p = dict(params, precision=prec)
+    g = SolverGPU(dims, p)
+    o = OracleSolver(dims, p, storage=ORACLE_STORAGE[prec])
+    for s in (g, o):
+        s.uploadFlags(flags)
+    a, b = g.readPopulations().astype(np.float64), o.readPopulations()
+    assert np.max(np.abs(a - b) / b) < TOL[prec] / 10
+
+
+def test_determinism():
+    """Two identical runs are bitwise equal (the FP64 force reduction has a fixed order)."""
+    from windtunnel_vis_amd import SolverGPU
+    dims, params, flags, sdf, wheels = make_case("d3_full")
+    outs = []
+    for _ in range(2):
+        g = SolverGPU(dims, dict(params, precision="f16"))
+        g.setWheels(wheels)
+        g.uploadFlags(flags)
+        g.uploadSdf(sdf)
+        st = g.stepAndSample(50)
+        outs.append((g.readPopulations(), g.sampleRaw(), st))
+        g.destroy()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])

@@ -1,0 +1,138 @@
+"""ctypes binding of libwt.so (the C ABI declared in include/wt.h).
+
+The library is loaded from the package's in-tree build (windtunnel_vis_amd/_lib/libwt.so). There
+is no fallback: if the library is missing or cannot be loaded, `load()` raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libwt.so")
+
+WT_OK = 0
+WT_E_ARG = -1
+WT_E_HIP = -2
+WT_E_NOMEM = -3
+WT_E_STATE = -4
+WT_E_NODEV = -5
+
+GROUND = {"freeslip": 0, "noslip": 1, "moving": 2}
+SIDES = {"freeslip": 0, "periodic": 1}
+PRECISION = {"f32": 0, "f16": 1, "bf16": 2}
+PRECISION_NAME = {v: k for k, v in PRECISION.items()}
+COLLISION = {"regularized": 0, "bgk": 1}
+CLOSURE = {"smagorinsky": 0, "mlp": 1}
+
+PRIM_ROUNDED_BOX = 1
+PRIM_CYLINDER = 2
+PRIM_SPHERE = 3
+PRIM_WEDGE = 4
+
+
+class Dims(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("u", C.c_double), ("nu", C.c_double), ("cs", C.c_double),
+        ("ground", C.c_int32), ("sides", C.c_int32),
+        ("sponge_nu", C.c_double), ("sponge_start", C.c_double), ("sponge_in", C.c_double),
+        ("tau_wall", C.c_double),
+        ("precision", C.c_int32), ("interp", C.c_int32), ("collision", C.c_int32), ("closure", C.c_int32),
+        ("ema_alpha", C.c_double),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double), ("max_u", C.c_double),
+        ("rho_ref", C.c_double), ("unstable", C.c_int32), ("pad", C.c_int32), ("step", C.c_int64),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("dims", Dims), ("n", C.c_int64), ("q", C.c_int32), ("precision", C.c_int32), ("step", C.c_int64),
+        ("boundary_pairs", C.c_int64), ("device_bytes", C.c_int64),
+    ]
+
+
+class VoxelInfo(C.Structure):
+    _fields_ = [("frontal", C.c_int64), ("solid_cells", C.c_int64), ("min", C.c_int32 * 3), ("max", C.c_int32 * 3)]
+
+
+class Prim(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("flags", C.c_int32), ("center", C.c_float * 3), ("rot", C.c_float * 9),
+        ("half", C.c_float * 3), ("radius", C.c_float), ("param", C.c_float),
+    ]
+
+
+# every exported symbol of include/wt.h with its signature: (restype, argtypes)
+_P = C.c_void_p
+SIGNATURES = {
+    "wt_create": (C.c_int, [C.POINTER(Dims), C.POINTER(Params), C.POINTER(_P)]),
+    "wt_destroy": (C.c_int, [_P]),
+    "wt_bytes_for": (C.c_int64, [C.POINTER(Dims)]),
+    "wt_info_get": (C.c_int, [_P, C.POINTER(Info)]),
+    "wt_last_error": (C.c_char_p, []),
+    "wt_set_params": (C.c_int, [_P, C.POINTER(Params), C.c_uint32]),
+    "wt_get_params": (C.c_int, [_P, C.POINTER(Params)]),
+    "wt_tau0": (C.c_double, [_P]),
+    "wt_reset": (C.c_int, [_P]),
+    "wt_clear_stats": (C.c_int, [_P]),
+    "wt_upload_flags": (C.c_int, [_P, _P]),
+    "wt_read_flags": (C.c_int, [_P, _P]),
+    "wt_upload_sdf": (C.c_int, [_P, _P]),
+    "wt_set_wheels": (C.c_int, [_P, _P, C.c_int32]),
+    "wt_voxelize_sdf": (C.c_int, [_P, C.POINTER(Prim), C.c_int32, C.POINTER(VoxelInfo)]),
+    "wt_voxelize_mesh": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P, _P, C.c_int32, C.POINTER(VoxelInfo)]),
+    "wt_step": (C.c_int, [_P, C.c_int32]),
+    "wt_step_and_sample": (C.c_int, [_P, C.c_int32, C.POINTER(Stats)]),
+    "wt_sample_raw": (C.c_int, [_P, _P]),
+    "wt_sync": (C.c_int, [_P]),
+    "wt_read_populations": (C.c_int, [_P, _P]),
+    "wt_upload_populations": (C.c_int, [_P, _P]),
+    "wt_read_macro": (C.c_int, [_P, _P, _P, _P]),
+    "wt_profile_enable": (C.c_int, [_P, C.c_int32]),
+    "wt_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "wt_work_split": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+}
+
+_lib = None
+
+
+class WtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("libwt error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libwt.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libwt.so is not built (%s); run `python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != WT_OK:
+        msg = _lib.wt_last_error().decode() if _lib is not None else ""
+        raise WtError(code, msg)
+    return code
+
+
+def ptr(a):
+    """Raw data pointer of a C-contiguous numpy array."""
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return C.c_void_p(a.ctypes.data)

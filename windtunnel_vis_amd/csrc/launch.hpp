@@ -1,0 +1,27 @@
+// Host-side launch interface between the C-ABI layer and the kernels.
+#pragma once
+#include "lbm.hpp"
+
+namespace wt {
+
+struct LaunchCtx {
+  hipStream_t stream;
+  KParams P;
+  int prec;                 // 0 f32, 1 f16, 2 bf16
+  const uint32_t* flags;
+  const uint32_t* sdf;
+  uint8_t* kind;
+  const KWheels* W;
+};
+
+void launch_init(const LaunchCtx& c, void* f);
+void launch_classify(const LaunchCtx& c, uint32_t* list, unsigned long long* counts);
+void launch_bulk(const LaunchCtx& c, const void* fin, void* fout);
+void launch_boundary(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, int64_t count);
+int64_t macro_blocks(int64_t n);
+void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, float* meanB, double* partials, double* result);
+void launch_unpack(const LaunchCtx& c, const void* f, float* out);
+void launch_pack(const LaunchCtx& c, const float* in, void* f);
+void launch_clear_stats(const LaunchCtx& c, void* meanA, float* meanB);
+
+}  // namespace wt

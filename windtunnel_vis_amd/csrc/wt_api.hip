@@ -1,0 +1,551 @@
+// C-ABI host layer (libwt.so): solver handle, memory ownership, stepping, sampling and readback.
+// Mirrors the member functions of SolverGPU (src/solver/SolverGPU.ts); each entry point is
+// declared and cited in include/wt.h.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wt.h"
+#include "launch.hpp"
+#include "voxel.hpp"
+
+using namespace wt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return fail(WT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct wt_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  wt_dims dims{};
+  int64_t n = 0;
+  int q = 0;
+  int prec = 0;
+  wt_params prm{};
+  KParams P{};
+  KWheels W{};
+  void* f[2] = {nullptr, nullptr};
+  uint32_t* flags = nullptr;
+  uint32_t* sdf = nullptr;
+  uint8_t* kind = nullptr;
+  uint32_t* blist = nullptr;
+  unsigned long long* counts = nullptr;
+  ushort4* vel = nullptr;
+  float4* meanA = nullptr;
+  float* meanB = nullptr;
+  double* partials = nullptr;
+  double* result = nullptr;
+  double* h_result = nullptr;  // pinned
+  int64_t nboundary = 0, nfast = 0, nsolid = 0;
+  int parity = 0;
+  bool pending_init = true;
+  int64_t step = 0;
+  int64_t bytes = 0;
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev;   // pairs: bulk start/stop, boundary start/stop per step
+  size_t ev_used = 0;
+  double acc_bulk = 0, acc_bnd = 0;
+  int64_t acc_launches = 0;
+
+  LaunchCtx ctx() const {
+    LaunchCtx c;
+    c.stream = stream;
+    c.P = P;
+    c.prec = prec;
+    c.flags = flags;
+    c.sdf = sdf;
+    c.kind = kind;
+    c.W = &W;
+    return c;
+  }
+};
+
+namespace {
+
+// SolverGPU.writeParams, SolverGPU.ts:160-187.
+void derive(wt_solver* s) {
+  KParams& P = s->P;
+  const wt_params& p = s->prm;
+  P.nx = s->dims.nx; P.ny = s->dims.ny; P.nz = s->dims.nz; P.q = s->q;
+  P.n = s->n;
+  P.tau0 = (float)(3.0 * p.nu + 0.5);
+  P.csmag2 = (float)(p.cs * p.cs);
+  P.uin = (float)p.u;
+  P.rhoIn = 1.0f;
+  P.ground = p.ground;
+  P.spongeStart = (float)(p.sponge_start * s->dims.nx);
+  P.spongeNu = (float)p.sponge_nu;
+  P.uGround = p.ground == WT_GROUND_MOVING ? (float)p.u : 0.0f;
+  P.emaAlpha = (float)p.ema_alpha;
+  P.uRef = (float)std::max(p.u, 1e-4);
+  P.refX = std::min(4, s->dims.nx - 1);
+  P.sideMode = p.sides == WT_SIDES_PERIODIC ? 1 : 0;
+  P.collision = p.collision == WT_COLL_BGK ? 1 : 0;
+  P.tauWall = (float)p.tau_wall;
+  P.spongeIn = (float)(p.sponge_in * s->dims.nx);
+  P.contactH = 2.0f;
+  P.interp = p.interp ? 1 : 0;
+  P.closure = p.closure;
+  P.divx.init((uint32_t)s->dims.nx);
+  P.divy.init((uint32_t)s->dims.ny);
+}
+
+int check_params(const wt_params* p) {
+  if (!(p->u >= 0.0 && p->u < 0.5)) return fail(WT_E_ARG, "params.u must be in [0, 0.5)");
+  if (!(p->nu > 0.0)) return fail(WT_E_ARG, "params.nu must be > 0");
+  if (!(p->cs >= 0.0)) return fail(WT_E_ARG, "params.cs must be >= 0");
+  if (p->ground < 0 || p->ground > 2) return fail(WT_E_ARG, "params.ground must be 0, 1 or 2");
+  if (p->sides < 0 || p->sides > 1) return fail(WT_E_ARG, "params.sides must be 0 or 1");
+  if (p->precision < 0 || p->precision > 2) return fail(WT_E_ARG, "params.precision must be 0, 1 or 2");
+  if (p->collision < 0 || p->collision > 1) return fail(WT_E_ARG, "params.collision must be 0 or 1");
+  if (p->closure < 0 || p->closure > 1) return fail(WT_E_ARG, "params.closure must be 0 or 1");
+  return WT_OK;
+}
+
+int run_init(wt_solver* s) {
+  LaunchCtx c = s->ctx();
+  launch_init(c, s->f[0]);
+  launch_init(c, s->f[1]);
+  HIP_TRY(hipGetLastError());
+  s->pending_init = false;
+  return WT_OK;
+}
+
+int reclassify(wt_solver* s) {
+  HIP_TRY(hipMemsetAsync(s->counts, 0, 4 * sizeof(unsigned long long), s->stream));
+  LaunchCtx c = s->ctx();
+  launch_classify(c, s->blist, s->counts);
+  HIP_TRY(hipGetLastError());
+  unsigned long long h[4];
+  HIP_TRY(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->nboundary = (int64_t)h[0];
+  s->nfast = (int64_t)h[1];
+  s->nsolid = (int64_t)h[2];
+  return WT_OK;
+}
+
+int do_steps(wt_solver* s, int steps) {
+  if (s->pending_init) {
+    int r = run_init(s);
+    if (r) return r;
+  }
+  LaunchCtx c = s->ctx();
+  for (int k = 0; k < steps; k++) {
+    const void* fin = s->f[s->parity];
+    void* fout = s->f[s->parity ^ 1];
+    if (s->prof) {
+      if (s->ev_used + 4 > s->ev.size()) {
+        size_t old = s->ev.size();
+        s->ev.resize(std::max<size_t>(64, old * 2));
+        for (size_t i = old; i < s->ev.size(); i++) HIP_TRY(hipEventCreate(&s->ev[i]));
+      }
+      hipEvent_t* e = &s->ev[s->ev_used];
+      s->ev_used += 4;
+      HIP_TRY(hipEventRecord(e[0], s->stream));
+      launch_bulk(c, fin, fout);
+      HIP_TRY(hipEventRecord(e[1], s->stream));
+      launch_boundary(c, fin, fout, s->blist, s->nboundary);
+      HIP_TRY(hipEventRecord(e[3], s->stream));
+      e[2] = e[1];
+    } else {
+      launch_bulk(c, fin, fout);
+      launch_boundary(c, fin, fout, s->blist, s->nboundary);
+    }
+    s->parity ^= 1;
+  }
+  HIP_TRY(hipGetLastError());
+  s->step += steps;
+  return WT_OK;
+}
+
+int do_sample(wt_solver* s) {
+  if (s->pending_init) {
+    int r = run_init(s);
+    if (r) return r;
+  }
+  LaunchCtx c = s->ctx();
+  launch_macro(c, s->f[s->parity], s->vel, s->meanA, s->meanB, s->partials, s->result);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(s->h_result, s->result, 8 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wt_last_error(void) { return g_err.c_str(); }
+
+int64_t wt_bytes_for(const wt_dims* d) {
+  if (!d) return 0;
+  const int64_t n = (int64_t)d->nx * d->ny * d->nz;
+  const int q = d->nz > 1 ? 19 : 9;
+  return n * (q * 8 + 4 + 20 + 32);
+}
+
+int wt_create(const wt_dims* dims, const wt_params* params, wt_solver** out) {
+  if (!dims || !params || !out) return fail(WT_E_ARG, "null argument");
+  *out = nullptr;
+  if (dims->nx < 2 || dims->ny < 2 || dims->nz < 1) return fail(WT_E_ARG, "dims must be nx >= 2, ny >= 2, nz >= 1");
+  const int64_t n = (int64_t)dims->nx * dims->ny * dims->nz;
+  if (n >= (1ll << 31)) return fail(WT_E_ARG, "grid too large: cell count must be below 2^31");
+  if (int r = check_params(params)) return r;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(WT_E_NODEV, "no HIP device available");
+  auto* s = new wt_solver();
+  HIP_TRY(hipGetDevice(&s->device));
+  s->dims = *dims;
+  s->n = n;
+  s->q = dims->nz > 1 ? 19 : 9;
+  s->prm = *params;
+  // half-width storage needs an even nx (pairs of cells per 32-bit word), else FP32, as SolverGPU.ts:69
+  s->prec = (params->precision != WT_PREC_F32 && dims->nx % 2 == 0) ? params->precision : WT_PREC_F32;
+  s->prm.precision = s->prec;
+  derive(s);
+  const int64_t sbytes = s->prec == WT_PREC_F32 ? 4 : 2;
+  const int64_t fbytes = (int64_t)s->q * n * sbytes;
+  const int64_t nblocks = macro_blocks(n);
+  auto alloc = [&](void** p, int64_t bytes) -> bool {
+    if (hipMalloc(p, (size_t)std::max<int64_t>(bytes, 256)) != hipSuccess) return false;
+    s->bytes += bytes;
+    return true;
+  };
+  // f[] gets one extra word of padding so pair loads at the very last cell stay in bounds
+  bool ok = alloc(&s->f[0], fbytes + 64) && alloc(&s->f[1], fbytes + 64) && alloc((void**)&s->flags, 4 * n) &&
+            alloc((void**)&s->sdf, 4 * n) && alloc((void**)&s->kind, n + 64) && alloc((void**)&s->blist, 4 * (n / 2 + 1)) &&
+            alloc((void**)&s->counts, 64) && alloc((void**)&s->vel, 8 * n) && alloc((void**)&s->meanA, 16 * n) &&
+            alloc((void**)&s->meanB, 4 * n) && alloc((void**)&s->partials, 64 * nblocks) && alloc((void**)&s->result, 64);
+  if (!ok) {
+    wt_destroy(s);
+    return fail(WT_E_NOMEM, "device allocation failed");
+  }
+  if (hipHostMalloc((void**)&s->h_result, 64, hipHostMallocDefault) != hipSuccess) {
+    wt_destroy(s);
+    return fail(WT_E_NOMEM, "pinned host allocation failed");
+  }
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    wt_destroy(s);
+    return fail(WT_E_HIP, "hipStreamCreate failed");
+  }
+  HIP_TRY(hipMemsetAsync(s->flags, 0, 4 * n, s->stream));
+  HIP_TRY(hipMemsetAsync(s->sdf, 0, 4 * n, s->stream));
+  HIP_TRY(hipMemsetAsync(s->f[0], 0, fbytes + 64, s->stream));
+  HIP_TRY(hipMemsetAsync(s->f[1], 0, fbytes + 64, s->stream));
+  HIP_TRY(hipMemsetAsync(s->kind, 0, n + 64, s->stream));
+  std::memset(&s->W, 0, sizeof(s->W));
+  if (int r = reclassify(s)) { wt_destroy(s); return r; }
+  if (int r = wt_reset(s)) { wt_destroy(s); return r; }
+  *out = s;
+  return WT_OK;
+}
+
+int wt_destroy(wt_solver* s) {
+  if (!s) return WT_OK;
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (auto e : s->ev) hipEventDestroy(e);
+  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->blist, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (s->h_result) hipHostFree(s->h_result);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return WT_OK;
+}
+
+int wt_info_get(const wt_solver* s, wt_info* out) {
+  if (!s || !out) return fail(WT_E_ARG, "null argument");
+  out->dims = s->dims;
+  out->n = s->n;
+  out->q = s->q;
+  out->precision = s->prec;
+  out->step = s->step;
+  out->boundary_pairs = s->nboundary;
+  out->device_bytes = s->bytes;
+  return WT_OK;
+}
+
+int wt_set_params(wt_solver* s, const wt_params* p, uint32_t mask) {
+  if (!s || !p) return fail(WT_E_ARG, "null argument");
+  wt_params np = s->prm;
+  if (mask & WT_P_U) np.u = p->u;
+  if (mask & WT_P_NU) np.nu = p->nu;
+  if (mask & WT_P_CS) np.cs = p->cs;
+  if (mask & WT_P_GROUND) np.ground = p->ground;
+  if (mask & WT_P_SIDES) np.sides = p->sides;
+  if (mask & WT_P_SPONGE_NU) np.sponge_nu = p->sponge_nu;
+  if (mask & WT_P_SPONGE_START) np.sponge_start = p->sponge_start;
+  if (mask & WT_P_SPONGE_IN) np.sponge_in = p->sponge_in;
+  if (mask & WT_P_TAU_WALL) np.tau_wall = p->tau_wall;
+  if (mask & WT_P_INTERP) np.interp = p->interp;
+  if (mask & WT_P_COLLISION) np.collision = p->collision;
+  if (mask & WT_P_CLOSURE) np.closure = p->closure;
+  if (mask & WT_P_EMA_ALPHA) np.ema_alpha = p->ema_alpha;
+  if (int r = check_params(&np)) return r;
+  np.precision = s->prec;  // fixed at construction
+  s->prm = np;
+  derive(s);
+  return WT_OK;
+}
+
+int wt_get_params(const wt_solver* s, wt_params* out) {
+  if (!s || !out) return fail(WT_E_ARG, "null argument");
+  *out = s->prm;
+  return WT_OK;
+}
+
+double wt_tau0(const wt_solver* s) { return s ? 3.0 * s->prm.nu + 0.5 : NAN; }
+
+int wt_reset(wt_solver* s) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  s->pending_init = true;
+  s->step = 0;
+  return wt_clear_stats(s);
+}
+
+int wt_clear_stats(wt_solver* s) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  LaunchCtx c = s->ctx();
+  launch_clear_stats(c, s->meanA, s->meanB);
+  HIP_TRY(hipGetLastError());
+  return WT_OK;
+}
+
+int wt_upload_flags(wt_solver* s, const uint32_t* flags) {
+  if (!s || !flags) return fail(WT_E_ARG, "null argument");
+  HIP_TRY(hipMemcpyAsync(s->flags, flags, 4 * s->n, hipMemcpyHostToDevice, s->stream));
+  return reclassify(s);
+}
+
+int wt_read_flags(wt_solver* s, uint32_t* out) {
+  if (!s || !out) return fail(WT_E_ARG, "null argument");
+  HIP_TRY(hipMemcpyAsync(out, s->flags, 4 * s->n, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+int wt_upload_sdf(wt_solver* s, const float* d) {
+  if (!s || !d) return fail(WT_E_ARG, "null argument");
+  std::vector<uint32_t> enc((size_t)s->n);
+  for (int64_t i = 0; i < s->n; i++) {
+    if (!(d[i] >= 0.0f)) { enc[i] = 0u; continue; }
+    uint32_t b;
+    std::memcpy(&b, &d[i], 4);
+    enc[i] = ~b;
+  }
+  HIP_TRY(hipMemcpyAsync(s->sdf, enc.data(), 4 * s->n, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+int wt_set_wheels(wt_solver* s, const float* w, int32_t count) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (count < 0 || (count > 0 && !w)) return fail(WT_E_ARG, "bad wheel list");
+  // the wheel table lives in the kernel argument block; wait for in-flight steps that use the old one
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::memset(&s->W, 0, sizeof(s->W));
+  const int m = std::min(count, 16);
+  s->W.count = m;
+  for (int k = 0; k < m; k++) {
+    const float* e = w + 8 * k;
+    s->W.w[2 * k] = make_float4(e[0], e[1], e[2], 1.0f / std::max(e[6], 1e-3f));
+    s->W.w[2 * k + 1] = make_float4(e[3], e[4], e[5], e[7]);
+  }
+  return WT_OK;
+}
+
+int wt_step(wt_solver* s, int32_t steps) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (steps < 0) return fail(WT_E_ARG, "steps must be >= 0");
+  return do_steps(s, steps);
+}
+
+int wt_step_and_sample(wt_solver* s, int32_t steps, wt_stats* out) {
+  if (!s || !out) return fail(WT_E_ARG, "null argument");
+  if (steps < 0) return fail(WT_E_ARG, "steps must be >= 0");
+  if (int r = do_steps(s, steps)) return r;
+  if (int r = do_sample(s)) return r;
+  const double* d = s->h_result;
+  // SolverGPU.ts:302-308
+  out->fx = d[0]; out->fy = d[1]; out->fz = d[2];
+  out->max_u = std::sqrt(std::max(d[3], 0.0));
+  out->rho_ref = d[5] > 0 ? d[4] / d[5] : 1.0;
+  out->unstable = (d[6] > 0 || !std::isfinite(d[0]) || d[3] > 0.25) ? 1 : 0;
+  out->pad = 0;
+  out->step = s->step;
+  return WT_OK;
+}
+
+int wt_sample_raw(wt_solver* s, double* out8) {
+  if (!s || !out8) return fail(WT_E_ARG, "null argument");
+  std::memcpy(out8, s->h_result, 8 * sizeof(double));
+  return WT_OK;
+}
+
+int wt_sync(wt_solver* s) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+int wt_read_populations(wt_solver* s, float* out) {
+  if (!s || !out) return fail(WT_E_ARG, "null argument");
+  if (s->pending_init) {
+    if (int r = run_init(s)) return r;
+  }
+  const int64_t cnt = (int64_t)s->q * s->n;
+  float* tmp = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&tmp, 4 * cnt, s->stream));
+  LaunchCtx c = s->ctx();
+  launch_unpack(c, s->f[s->parity], tmp);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, tmp, 4 * cnt, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipFreeAsync(tmp, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+int wt_upload_populations(wt_solver* s, const float* f) {
+  if (!s || !f) return fail(WT_E_ARG, "null argument");
+  if (s->pending_init) {
+    if (int r = run_init(s)) return r;
+  }
+  const int64_t cnt = (int64_t)s->q * s->n;
+  float* tmp = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&tmp, 4 * cnt, s->stream));
+  HIP_TRY(hipMemcpyAsync(tmp, f, 4 * cnt, hipMemcpyHostToDevice, s->stream));
+  LaunchCtx c = s->ctx();
+  launch_pack(c, tmp, s->f[s->parity]);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipFreeAsync(tmp, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+int wt_read_macro(wt_solver* s, float* vel4, float* mean_a4, float* mean_b) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (vel4) {
+    std::vector<uint16_t> h((size_t)(4 * s->n));
+    HIP_TRY(hipMemcpyAsync(h.data(), s->vel, 8 * s->n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int64_t i = 0; i < 4 * s->n; i++) {
+      // IEEE half -> float on the host
+      const uint32_t hv = h[i];
+      const uint32_t sign = (hv & 0x8000u) << 16, ex = (hv >> 10) & 0x1f, man = hv & 0x3ff;
+      float v;
+      if (ex == 0) v = std::ldexp((float)man, -24);
+      else if (ex == 31) v = man ? NAN : INFINITY;
+      else v = std::ldexp((float)(man | 0x400), (int)ex - 25);
+      uint32_t b;
+      std::memcpy(&b, &v, 4);
+      b |= sign;
+      std::memcpy(&vel4[i], &b, 4);
+    }
+  }
+  if (mean_a4) HIP_TRY(hipMemcpyAsync(mean_a4, s->meanA, 16 * s->n, hipMemcpyDeviceToHost, s->stream));
+  if (mean_b) HIP_TRY(hipMemcpyAsync(mean_b, s->meanB, 4 * s->n, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+int wt_profile_enable(wt_solver* s, int32_t on) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->prof = on != 0;
+  s->ev_used = 0;
+  s->acc_bulk = s->acc_bnd = 0;
+  s->acc_launches = 0;
+  return WT_OK;
+}
+
+int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* boundary_ms) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (size_t i = 0; i + 4 <= s->ev_used; i += 4) {
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, s->ev[i], s->ev[i + 1]));
+    HIP_TRY(hipEventElapsedTime(&b, s->ev[i + 2], s->ev[i + 3]));
+    s->acc_bulk += a;
+    s->acc_bnd += b;
+    s->acc_launches++;
+  }
+  s->ev_used = 0;
+  if (bulk_ms) *bulk_ms = s->acc_bulk;
+  if (launches) *launches = s->acc_launches;
+  if (boundary_ms) *boundary_ms = s->acc_bnd;
+  s->acc_bulk = s->acc_bnd = 0;
+  s->acc_launches = 0;
+  return WT_OK;
+}
+
+int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cells) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (bulk_cells) *bulk_cells = 2 * s->nfast;
+  if (boundary_cells) *boundary_cells = 2 * s->nboundary;
+  return WT_OK;
+}
+
+// ---- voxelizers ----------------------------------------------------------------------------
+
+int wt_voxelize_sdf(wt_solver* s, const wt_prim* prims, int32_t count, wt_voxel_info* info) {
+  if (!s || (count > 0 && !prims)) return fail(WT_E_ARG, "null argument");
+  if (count < 0 || count > WT_MAX_PRIMS) return fail(WT_E_ARG, "primitive count must be in [0, 64]");
+  VoxCtx v{s->stream, s->dims.nx, s->dims.ny, s->dims.nz, s->n, s->flags, s->sdf, &s->W};
+  int64_t h[8];
+  if (int r = voxelize_sdf(v, prims, count, h)) return fail(WT_E_HIP, "analytic voxelizer failed");
+  if (int r = reclassify(s)) return r;
+  if (info) {
+    info->frontal = h[0];
+    info->solid_cells = h[1];
+    for (int a = 0; a < 3; a++) {
+      info->min[a] = h[1] > 0 ? (int32_t)h[2 + a] : 0;
+      info->max[a] = h[1] > 0 ? (int32_t)h[5 + a] + 1 : 0;
+    }
+  }
+  return WT_OK;
+}
+
+int wt_voxelize_mesh(wt_solver* s, const float* positions, int32_t n_vertices, const uint32_t* indices, int32_t n_triangles,
+                     const float* matrix16, const float* wheels, int32_t n_wheels, wt_voxel_info* info) {
+  if (!s || !matrix16 || (n_vertices > 0 && !positions) || (n_triangles > 0 && !indices)) return fail(WT_E_ARG, "null argument");
+  if (n_vertices < 0 || n_triangles < 0) return fail(WT_E_ARG, "negative size");
+  for (int64_t i = 0; i < 3 * (int64_t)n_triangles; i++)
+    if (indices[i] >= (uint32_t)n_vertices) return fail(WT_E_ARG, "triangle index out of range");
+  // VoxelizerGPU.ts:49: the call always replaces the solver's wheel list (an empty list clears it)
+  if (int r = wt_set_wheels(s, wheels, wheels ? n_wheels : 0)) return r;
+  VoxCtx v{s->stream, s->dims.nx, s->dims.ny, s->dims.nz, s->n, s->flags, s->sdf, &s->W};
+  int64_t h[8];
+  if (int r = voxelize_mesh(v, positions, n_vertices, indices, n_triangles, matrix16, h)) return fail(WT_E_HIP, g_err.empty() ? "mesh voxelizer failed" : g_err);
+  if (int r = reclassify(s)) return r;
+  if (info) {
+    info->frontal = h[0];
+    info->solid_cells = h[1];
+    for (int a = 0; a < 3; a++) {
+      info->min[a] = h[1] > 0 ? (int32_t)h[2 + a] : 0;
+      info->max[a] = h[1] > 0 ? (int32_t)h[5 + a] + 1 : 0;
+    }
+  }
+  return WT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,249 @@
+"""Python host of the MI355X solver: the reference's `SolverGPU` surface (src/solver/SolverGPU.ts)
+and `LbmSolver` contract (src/backend/types.ts:7-18) over the C ABI.
+
+Method names follow the reference (camelCase) so host code and tests read like the reference's.
+Parameters are given as a dict shaped like `SolverParams` (src/solver/types.ts:3-29).
+The reference's stepAndSample is a Promise; here it is a blocking call returning the stats.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+D2Q9_C = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0],
+                   [1, 1, 0], [-1, 1, 0], [-1, -1, 0], [1, -1, 0]], dtype=np.int64)
+D2Q9_W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+D3Q19_C = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+                    [1, 1, 0], [-1, -1, 0], [1, 0, 1], [-1, 0, -1], [0, 1, 1], [0, -1, -1],
+                    [1, -1, 0], [-1, 1, 0], [1, 0, -1], [-1, 0, 1], [0, 1, -1], [0, -1, 1]], dtype=np.int64)
+D3Q19_W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+@dataclass
+class Lattice:
+    """Velocity set (src/solver/lattice.ts:5-60)."""
+    name: str
+    dim: int
+    q: int
+    c: np.ndarray
+    w: np.ndarray
+
+
+D2Q9 = Lattice("D2Q9", 2, 9, D2Q9_C, D2Q9_W)
+D3Q19 = Lattice("D3Q19", 3, 19, D3Q19_C, D3Q19_W)
+
+
+@dataclass
+class SolverStats:
+    """SolverStats (src/solver/types.ts:31-42)."""
+    fx: float
+    fy: float
+    fz: float
+    maxU: float
+    rhoRef: float
+    unstable: bool
+    step: int
+
+
+DEFAULTS = dict(u=0.05, nu=0.01, cs=0.1, ground="freeslip", sides="freeslip", spongeNu=0.0, spongeStart=0.9,
+                spongeIn=0.0, tauWall=0.5, precision="f32", interp=True, collision="regularized",
+                closure="smagorinsky", emaAlpha=0.0)
+
+_FIELDS = [  # (SolverParams key, struct field, mask bit, encoder)
+    ("u", "u", 0, float), ("nu", "nu", 1, float), ("cs", "cs", 2, float),
+    ("ground", "ground", 3, lambda v: N.GROUND[v]), ("sides", "sides", 4, lambda v: N.SIDES[v]),
+    ("spongeNu", "sponge_nu", 5, float), ("spongeStart", "sponge_start", 6, float),
+    ("spongeIn", "sponge_in", 7, float), ("tauWall", "tau_wall", 8, float),
+    ("interp", "interp", 9, lambda v: 1 if v else 0),
+    ("collision", "collision", 10, lambda v: N.COLLISION[v]), ("closure", "closure", 11, lambda v: N.CLOSURE[v]),
+    ("emaAlpha", "ema_alpha", 12, float),
+]
+
+
+def _params_struct(p):
+    s = N.Params()
+    mask = 0
+    for key, field, bit, enc in _FIELDS:
+        if key in p and p[key] is not None:
+            setattr(s, field, enc(p[key]))
+            mask |= 1 << bit
+    s.precision = N.PRECISION[p.get("precision") or "f32"]
+    return s, mask
+
+
+class SolverGPU:
+    """GPU lattice-Boltzmann solver (D2Q9 when nz == 1, D3Q19 otherwise), SolverGPU.ts:12-444."""
+
+    def __init__(self, dims, params):
+        self._lib = N.load()
+        if isinstance(dims, dict):
+            nx, ny, nz = dims["nx"], dims["ny"], dims["nz"]
+        else:
+            nx, ny, nz = dims
+        self.dims = dict(nx=int(nx), ny=int(ny), nz=int(nz))
+        self.is3D = nz > 1
+        self.lattice = D3Q19 if self.is3D else D2Q9
+        self.n = nx * ny * nz
+        p = dict(DEFAULTS)
+        p.update({k: v for k, v in params.items() if v is not None})
+        self._p = p
+        ps, _ = _params_struct(p)
+        d = N.Dims(nx, ny, nz)
+        h = N.C.c_void_p()
+        N.check(self._lib.wt_create(N.C.byref(d), N.C.byref(ps), N.C.byref(h)))
+        self._h = h
+        info = self.info()
+        self.precision = N.PRECISION_NAME[info.precision]
+        self.packed = self.precision != "f32"
+
+    # -- lifecycle --------------------------------------------------------------------------
+    def destroy(self):
+        if getattr(self, "_h", None):
+            self._lib.wt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @staticmethod
+    def bytesFor(dims):
+        d = N.Dims(dims["nx"], dims["ny"], dims["nz"])
+        return int(N.load().wt_bytes_for(N.C.byref(d)))
+
+    def info(self):
+        i = N.Info()
+        N.check(self._lib.wt_info_get(self._h, N.C.byref(i)))
+        return i
+
+    @property
+    def stepCount(self):
+        return int(self.info().step)
+
+    @property
+    def params_(self):
+        return dict(self._p)
+
+    @property
+    def tau0(self):
+        return float(self._lib.wt_tau0(self._h))
+
+    def setParams(self, p):
+        self._p.update(p)
+        ps, mask = _params_struct(p)
+        N.check(self._lib.wt_set_params(self._h, N.C.byref(ps), mask))
+
+    def reset(self):
+        N.check(self._lib.wt_reset(self._h))
+
+    def clearStats(self):
+        N.check(self._lib.wt_clear_stats(self._h))
+
+    # -- geometry ---------------------------------------------------------------------------
+    def uploadFlags(self, flags):
+        a = np.ascontiguousarray(flags, dtype=np.uint32).reshape(-1)
+        assert a.size == self.n, "flags must have n entries"
+        N.check(self._lib.wt_upload_flags(self._h, N.ptr(a)))
+
+    def readFlags(self):
+        out = np.empty(self.n, dtype=np.uint32)
+        N.check(self._lib.wt_read_flags(self._h, N.ptr(out)))
+        return out
+
+    def uploadSdf(self, d):
+        a = np.ascontiguousarray(d, dtype=np.float32).reshape(-1)
+        assert a.size == self.n, "sdf must have n entries"
+        N.check(self._lib.wt_upload_sdf(self._h, N.ptr(a)))
+
+    def setWheels(self, wheels):
+        """wheels: list of dicts {c: [x,y,z], axis: [x,y,z], r, hw} in grid space (SolverGPU.ts:325)."""
+        a = wheel_array(wheels)
+        N.check(self._lib.wt_set_wheels(self._h, N.ptr(a) if len(a) else None, len(a) // 8))
+
+    def voxelizeSdf(self, prims):
+        """Analytic-SDF voxelizer (a list of N.Prim); returns VoxelInfo as a dict."""
+        arr = (N.Prim * max(len(prims), 1))(*prims)
+        vi = N.VoxelInfo()
+        N.check(self._lib.wt_voxelize_sdf(self._h, arr, len(prims), N.C.byref(vi)))
+        return voxel_info(vi)
+
+    def voxelizeMesh(self, positions, indices, matrix, wheels=()):
+        """VoxelizerGPU.voxelize (src/voxelize/VoxelizerGPU.ts:48-151). matrix: 16 floats, column-major."""
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+        m = np.ascontiguousarray(matrix, dtype=np.float32).reshape(16)
+        wa = wheel_array(wheels)
+        vi = N.VoxelInfo()
+        N.check(self._lib.wt_voxelize_mesh(self._h, N.ptr(pos), pos.size // 3, N.ptr(idx), idx.size // 3, N.ptr(m),
+                                           N.ptr(wa) if len(wa) else None, len(wa) // 8, N.C.byref(vi)))
+        return voxel_info(vi)
+
+    # -- stepping ---------------------------------------------------------------------------
+    def encodeSteps(self, steps):
+        """Enqueue `steps` steps (asynchronous), SolverGPU.ts:204-222."""
+        N.check(self._lib.wt_step(self._h, int(steps)))
+
+    step = encodeSteps
+
+    def stepAndSample(self, steps):
+        """Run `steps` steps, then macro + force reduction, and return SolverStats (SolverGPU.ts:292-309)."""
+        st = N.Stats()
+        N.check(self._lib.wt_step_and_sample(self._h, int(steps), N.C.byref(st)))
+        return SolverStats(st.fx, st.fy, st.fz, st.max_u, st.rho_ref, bool(st.unstable), int(st.step))
+
+    def sampleRaw(self):
+        out = np.zeros(8, dtype=np.float64)
+        N.check(self._lib.wt_sample_raw(self._h, N.ptr(out)))
+        return out
+
+    def sync(self):
+        N.check(self._lib.wt_sync(self._h))
+
+    # -- readback ---------------------------------------------------------------------------
+    def readPopulations(self):
+        """Current populations, unpacked to full f_i, shape (q, n) float32."""
+        out = np.empty((self.lattice.q, self.n), dtype=np.float32)
+        N.check(self._lib.wt_read_populations(self._h, N.ptr(out)))
+        return out
+
+    def uploadPopulations(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float32).reshape(self.lattice.q, self.n)
+        N.check(self._lib.wt_upload_populations(self._h, N.ptr(a)))
+
+    def readMacro(self):
+        vel = np.empty((self.n, 4), dtype=np.float32)
+        ma = np.empty((self.n, 4), dtype=np.float32)
+        mb = np.empty(self.n, dtype=np.float32)
+        N.check(self._lib.wt_read_macro(self._h, N.ptr(vel), N.ptr(ma), N.ptr(mb)))
+        return vel, ma, mb
+
+    # -- profiling --------------------------------------------------------------------------
+    def profileEnable(self, on=True):
+        N.check(self._lib.wt_profile_enable(self._h, 1 if on else 0))
+
+    def profileRead(self):
+        b = N.C.c_double()
+        k = N.C.c_int64()
+        bd = N.C.c_double()
+        N.check(self._lib.wt_profile_read(self._h, N.C.byref(b), N.C.byref(k), N.C.byref(bd)))
+        return b.value, k.value, bd.value
+
+    def workSplit(self):
+        a = N.C.c_int64()
+        b = N.C.c_int64()
+        N.check(self._lib.wt_work_split(self._h, N.C.byref(a), N.C.byref(b)))
+        return a.value, b.value
+
+
+def wheel_array(wheels):
+    out = []
+    for w in list(wheels)[:16] if wheels is not None else []:
+        out += [w["c"][0], w["c"][1], w["c"][2], w["axis"][0], w["axis"][1], w["axis"][2], w["r"], w["hw"]]
+    return np.array(out, dtype=np.float32)
+
+
+def voxel_info(vi):
+    return dict(frontal=int(vi.frontal), solidCells=int(vi.solid_cells), min=list(vi.min), max=list(vi.max))
