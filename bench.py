@@ -66,17 +66,17 @@ def run_gpu(dims, steps, warmup, precision="f16"):
     s.step(steps)
     s.sync()
     t1 = time.perf_counter()
-    bulk_ms, launches, bnd_ms = s.profileRead()
+    bulk_ms, launches, wall_ms = s.profileRead()
     s.profileEnable(False)
     st = s.stepAndSample(0)
-    bulk_cells, bnd_cells = s.workSplit()
+    bulk_cells, wall_cells = s.workSplit()
     wall = t1 - t0
     mlups = n * steps / wall / 1e6
     bulk_avg_s = bulk_ms / 1e3 / max(launches, 1)
     bulk_bytes = bulk_cells * 2 * q * sbytes
     achieved = bulk_bytes / bulk_avg_s / 1e9
     return dict(solver=s, body=body, info=info, mlups=mlups, wall=wall, bulk_avg_ms=bulk_avg_s * 1e3,
-                bnd_avg_ms=bnd_ms / max(launches, 1), bulk_cells=bulk_cells, bnd_cells=bnd_cells, bulk_bytes=bulk_bytes,
+                wall_avg_ms=wall_ms / max(launches, 1), bulk_cells=bulk_cells, wall_cells=wall_cells, bulk_bytes=bulk_bytes,
                 achieved_gbs=achieved, unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes)
 
 
@@ -152,7 +152,7 @@ def main():
                        "kernel": "k_bulk<D3Q19,SF16>", "bytes_per_launch": r["bulk_bytes"],
                        "avg_launch_ms": round(r["bulk_avg_ms"], 5), "cells_per_launch": r["bulk_cells"]}
     out["step_roofline_frac"] = round(r["mlups"] * 1e6 * 2 * r["q"] * r["sbytes"] / (HBM_PEAK_GBS * 1e9), 4)
-    out["boundary_kernel_ms"] = round(r["bnd_avg_ms"], 5)
+    out["wall_kernel"] = {"avg_launch_ms": round(r["wall_avg_ms"], 5), "cells": r["wall_cells"]}
     out["unstable"] = bool(r["unstable"])
     if not args.no_physics:
         ph = physics(r["solver"], r["body"], r["info"], 0.08)

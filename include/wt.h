@@ -103,7 +103,7 @@ typedef struct {
   int32_t q;           /* 9 or 19 */
   int32_t precision;   /* storage actually used (WT_PREC_*) */
   int64_t step;        /* stepCount */
-  int64_t boundary_pairs; /* x-pairs of cells routed to the boundary kernel */
+  int64_t wall_cells;  /* cells next to the body, updated by the wall kernel */
   int64_t device_bytes;   /* device memory held by the handle */
 } wt_info;
 
@@ -207,12 +207,12 @@ int wt_read_macro(wt_solver* s, float* vel4, float* mean_a4, float* mean_b);
 
 /* When enabled, every step records HIP events around its bulk kernel on the handle's stream.
  * wt_profile_read synchronises and returns the summed bulk-kernel time (ms), the number of bulk
- * launches timed, and the summed boundary-kernel time; then clears the accumulators. */
+ * launches timed, and the summed wall-kernel time; then clears the accumulators. */
 int wt_profile_enable(wt_solver* s, int32_t on);
 int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* boundary_ms);
 
-/* Number of cells the bulk kernel updates per step (interior fluid + solid pairs) and the number
- * the boundary kernel updates. */
+/* Number of fluid cells the bulk kernel updates per step and the number of cells the wall kernel
+ * updates (cells next to the body). */
 int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cells);
 
 #ifdef __cplusplus

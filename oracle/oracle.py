@@ -165,16 +165,22 @@ class OracleSolver:
 
     encodeSteps = step
 
-    def sampleRaw(self):
+    def sample(self):
+        """Run the macro pass + reduction (updates the EMA statistics) and return the raw result."""
         out = np.zeros(8, dtype=np.float64)
         self._lib.or_sample(self._h, _ptr(out))
+        self._last = out
         return out
+
+    def sampleRaw(self):
+        """The raw result of the last sample (like wt_sample_raw)."""
+        return getattr(self, "_last", np.zeros(8))
 
     def stepAndSample(self, steps):
         """Same decode as SolverGPU.ts:302-308."""
         from windtunnel_vis_amd.solver import SolverStats
         self.step(steps)
-        d = self.sampleRaw()
+        d = self.sample()
         return SolverStats(d[0], d[1], d[2], float(np.sqrt(max(d[3], 0.0))), d[4] / d[5] if d[5] > 0 else 1.0,
                            bool(d[6] > 0 or not np.isfinite(d[0]) or d[3] > 0.25), self.stepCount)
 

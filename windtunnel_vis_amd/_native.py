@@ -54,7 +54,7 @@ class Stats(C.Structure):
 class Info(C.Structure):
     _fields_ = [
         ("dims", Dims), ("n", C.c_int64), ("q", C.c_int32), ("precision", C.c_int32), ("step", C.c_int64),
-        ("boundary_pairs", C.c_int64), ("device_bytes", C.c_int64),
+        ("wall_cells", C.c_int64), ("device_bytes", C.c_int64),
     ]
 
 

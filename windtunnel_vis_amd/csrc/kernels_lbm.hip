@@ -1,5 +1,5 @@
 // Lattice-Boltzmann kernels for gfx950: init, cell classification, the bulk fused
-// pull-stream + collide kernel, the boundary kernel, and the macro / momentum-exchange reduction.
+// pull-stream + collide kernel, the wall kernel, and the macro / momentum-exchange reduction.
 #include "lbm.hpp"
 #include "launch.hpp"
 
@@ -27,29 +27,41 @@ __global__ __launch_bounds__(BLOCK) void k_init(typename S::T* __restrict__ fout
 }
 
 // ---------------------------------------------------------------------------------------------
-// classify: kind byte per cell (see lbm.hpp) and the list of x-pairs that need the boundary path.
-// A fluid cell is "boundary" when any pull source x - c_i leaves the domain or is solid.
+// Classification. A fluid cell is WALL when the reference's plain pull branch (wgslLbm.ts:172) of
+// some direction reads a solid cell; inlet, ground, mirror and periodic pulls never test solids.
 // ---------------------------------------------------------------------------------------------
+template <class L>
+__device__ inline uint32_t solid_mask(const uint32_t* __restrict__ flags, int ix, int iy, int iz, const KParams& P) {
+  uint32_t m = 0u;
+#pragma unroll
+  for (int i = 1; i < L::Q; i++) {
+    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
+    int sx = ix - cx;
+    const int sy = iy - cy, sz = L::DIM == 3 ? iz - cz : 0;
+    if (cx < 0) sx = min(sx, P.nx - 1);
+    if (cx > 0 && sx < 0) continue;
+    if (sy < 0 || sy >= P.ny || sz < 0 || sz >= P.nz) continue;
+    if (flags[(int64_t)sx + (int64_t)P.nx * ((int64_t)sy + (int64_t)P.ny * sz)] != 0u) m |= 1u << i;
+  }
+  return m;
+}
+
+__device__ inline void coords(int64_t idx, const KParams& P, int& ix, int& iy, int& iz) {
+  const uint32_t row = P.divx.div((uint32_t)idx);
+  ix = (int)((uint32_t)idx - row * (uint32_t)P.nx);
+  const uint32_t zz = P.divy.div(row);
+  iy = (int)(row - zz * (uint32_t)P.ny);
+  iz = (int)zz;
+}
+
 template <class L>
 __global__ __launch_bounds__(BLOCK) void k_classify(const uint32_t* __restrict__ flags, uint8_t* __restrict__ kind, KParams P) {
   const int64_t idx = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (idx >= P.n) return;
-  const uint32_t row = P.divx.div((uint32_t)idx);
-  const int ix = (int)((uint32_t)idx - row * (uint32_t)P.nx);
-  const uint32_t zz = P.divy.div(row);
-  const int iy = (int)(row - zz * (uint32_t)P.ny);
-  const int iz = (int)zz;
+  int ix, iy, iz;
+  coords(idx, P, ix, iy, iz);
   if (flags[idx] != 0u) { kind[idx] = KIND_SOLID; return; }
-  bool b = ix == 0 || ix == P.nx - 1 || iy == 0 || iy == P.ny - 1;
-  if (L::DIM == 3) b = b || iz == 0 || iz == P.nz - 1;
-  if (!b) {
-#pragma unroll
-    for (int i = 1; i < L::Q; i++) {
-      const int64_t s = idx - (L::c(i, 0) + (int64_t)P.nx * (L::c(i, 1) + (int64_t)P.ny * L::c(i, 2)));
-      if (flags[s] != 0u) b = true;
-    }
-  }
-  uint8_t k = b ? KIND_BOUNDARY : KIND_FLUID;
+  uint8_t k = solid_mask<L>(flags, ix, iy, iz, P) ? KIND_WALL : KIND_FLUID;
   if (iy == 0) {  // tyre contact patch scan, wgslLbm.ts:276-287 (contactH = 2, SolverGPU.ts:183)
     const int r = (int)P.contactH;
     bool hold = false;
@@ -67,92 +79,173 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const uint32_t* __restrict__
   kind[idx] = k;
 }
 
-// Pairs (2p, 2p+1) with a boundary cell go to the list; counts[0] = list length,
-// counts[1] = fast pairs, counts[2] = all-solid pairs.
-__global__ __launch_bounds__(BLOCK) void k_collect(const uint16_t* __restrict__ kind2, int64_t npairs, uint32_t* __restrict__ list,
-                                                   unsigned long long* __restrict__ counts) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  bool bnd = false, fast = false, sol = false;
-  if (p < npairs) {
-    const uint16_t k = kind2[p];
-    fast = (k == 0);
-    sol = (k == 0x0101u);
-    bnd = !fast && !sol;
+// Groups of CPT cells along x (the bulk kernel's unit) that contain a WALL cell, or mix solid and
+// fluid, go to the wall kernel; their cells are listed with their solid masks.
+// counts[0] = listed cells, counts[1] = bulk fluid cells, counts[2] = bulk solid cells.
+template <class L, int CPT>
+__global__ __launch_bounds__(BLOCK) void k_collect(const uint8_t* __restrict__ kind, const uint32_t* __restrict__ flags,
+                                                   uint32_t* __restrict__ list, uint32_t* __restrict__ masks,
+                                                   unsigned long long* __restrict__ counts, KParams P) {
+  const int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x;  // group index
+  const int64_t ngroups = (P.n + CPT - 1) / CPT;
+  int nl = 0, nf = 0, ns = 0;
+  const int64_t c0 = g * CPT;
+  if (g < ngroups) {
+    bool anyw = false, anys = false, alls = true;
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      const int kj = (c0 + j < P.n) ? kind[c0 + j] : KIND_SOLID;
+      anyw |= (kj & KIND_WALL) != 0;
+      anys |= (kj & KIND_SOLID) != 0;
+      alls &= (kj & KIND_SOLID) != 0;
+    }
+    // odd cell counts: a group that runs past the end is always listed
+    const bool tail = c0 + CPT > P.n;
+    if (anyw || (anys && !alls) || tail) nl = (int)min<int64_t>(CPT, P.n - c0);
+    else if (alls) ns = CPT;
+    else nf = CPT;
   }
   // wave-aggregated append
-  const unsigned long long mb = __ballot(bnd);
-  const unsigned long long mf = __ballot(fast);
-  const unsigned long long ms = __ballot(sol);
   const int lane = threadIdx.x & 63;
-  uint32_t base = 0;
+  int incl = nl;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  const int total = __shfl(incl, 63);
+  unsigned long long base = 0;
+  int tf = nf, ts = ns;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { tf += __shfl_xor(tf, o); ts += __shfl_xor(ts, o); }
   if (lane == 0) {
-    if (mb) base = (uint32_t)atomicAdd(&counts[0], (unsigned long long)__popcll(mb));
-    if (mf) atomicAdd(&counts[1], (unsigned long long)__popcll(mf));
-    if (ms) atomicAdd(&counts[2], (unsigned long long)__popcll(ms));
+    if (total) base = atomicAdd(&counts[0], (unsigned long long)total);
+    if (tf) atomicAdd(&counts[1], (unsigned long long)tf);
+    if (ts) atomicAdd(&counts[2], (unsigned long long)ts);
   }
   base = __shfl(base, 0);
-  if (bnd) {
-    const int rank = __popcll(mb & ((1ull << lane) - 1ull));
-    list[base + rank] = (uint32_t)p;
+  if (nl) {
+    const uint32_t pos = (uint32_t)(base + incl - nl);
+    for (int j = 0; j < nl; j++) {
+      const int64_t c = c0 + j;
+      int ix, iy, iz;
+      coords(c, P, ix, iy, iz);
+      list[pos + j] = (uint32_t)c;
+      masks[pos + j] = (kind[c] & KIND_SOLID) ? 0u : solid_mask<L>(flags, ix, iy, iz, P);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Bulk kernel: one thread per x-pair of cells. Pairs whose two cells are interior fluid take the
-// fast path (plain pulls from in-domain fluid neighbours, then collision); all-solid pairs write
-// the rest state; pairs with a boundary cell are left to the boundary kernel.
-// HBM traffic per fast cell: Q loads + Q stores of S::BYTES (the algorithmic 2 Q s), plus one
-// kind byte.
+// Bulk kernel. Block = 4 waves; each wave owns a chunk of 64 * CPT cells of one x-row, so the row
+// (y, z) — and with it every face condition (ground, mirrors, periodic z, y-z edges) — is uniform
+// across the wave and is resolved with scalar arithmetic. For each direction the kernel picks the
+// source direction and source row once per wave, then every lane loads CPT consecutive cells
+// (x shifted by -c_x; unaligned dword loads for FP16/BF16 pairs). Inlet (x = 0) and outlet clamp
+// (x = nx - 1) are per-lane fix-ups. Groups holding a WALL cell are left to the wall kernel.
+// HBM traffic per cell: Q loads + Q stores of S::BYTES (the algorithmic 2 Q s) + 1 kind byte.
 // ---------------------------------------------------------------------------------------------
-template <class L, class S>
+template <class L, class S, int CPT>
 __global__ __launch_bounds__(BLOCK) void k_bulk(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                                const uint16_t* __restrict__ kind2, KParams P) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+                                                const uint8_t* __restrict__ kind, KParams P) {
+  using T = typename S::T;
+  constexpr int Q = L::Q;
+  constexpr bool D3 = L::DIM == 3;
+  const uint32_t ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const uint32_t row = blockIdx.x * 4u + ty;
+  const uint32_t nrows = (uint32_t)P.ny * (uint32_t)P.nz;
+  if (row >= nrows) return;
+  const int x0 = (int)(blockIdx.y * 64u + threadIdx.x) * CPT;
+  const int NX = P.nx, NY = P.ny, NZ = P.nz;
+  if (x0 >= NX) return;
   const int64_t n = P.n;
-  if (p >= (n >> 1)) return;
-  const uint16_t k = kind2[p];
-  const int64_t c0 = 2 * p;
-  if (k == 0x0101u) {
+  const int iz = (int)(row / (uint32_t)NY);
+  const int iy = (int)row - iz * NY;
+  const int64_t rowoff = (int64_t)row * NX;
+  const int64_t c0 = rowoff + x0;
+  uint32_t k;
+  if constexpr (CPT == 2) k = *(const uint16_t*)(kind + c0);
+  else k = kind[c0];
+  constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
+  constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
+  if (k == SOLID_ALL) {
 #pragma unroll
-    for (int i = 0; i < L::Q; i++) S::st2(fout, (int64_t)i * n + c0, L::w(i), L::w(i), L::w(i));
+    for (int i = 0; i < Q; i++) {
+      T* o = fout + (int64_t)i * n + c0;
+      if constexpr (CPT == 2) S::st2_dev(o, L::w(i), 0.0f, 0.0f);
+      else S::st_dev(o, 0, L::w(i), 0.0f);
+    }
     return;
   }
-  if (k != 0) return;
-  float fa[L::Q], fb[L::Q];
+  if (k & SKIP) return;
+  const bool yLo = iy == 0, yHi = iy == NY - 1;
+  const bool zLo = D3 && iz == 0, zHi = D3 && iz == NZ - 1;
+  const bool gwall = yLo && P.ground != 0;
+  const float beltA = (P.ground == 2 && !(k & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
+  const float beltB = (P.ground == 2 && !((k >> 8) & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
+  const float U = P.uin;
+  float fa[Q], fb[Q];
 #pragma unroll
-  for (int i = 0; i < L::Q; i++) {
-    const int64_t off = L::c(i, 0) + (int64_t)P.nx * (L::c(i, 1) + (int64_t)P.ny * L::c(i, 2));
-    const int64_t e = (int64_t)i * n + c0 - off;
-    if (L::c(i, 0) == 0) S::template ld2<true>(fin, e, L::w(i), fa[i], fb[i]);
-    else S::template ld2<false>(fin, e, L::w(i), fa[i], fb[i]);
+  for (int i = 0; i < Q; i++) {
+    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
+    // ---- source direction and row (wave-uniform) ----
+    const bool yOut = (cy > 0 && yLo) || (cy < 0 && yHi);
+    const bool zOut = (cz > 0 && zLo) || (cz < 0 && zHi);
+    const bool grnd = cy > 0 && yOut && P.ground != 0;
+    int d, sy, sz, xs;
+    if (i == 0) { d = 0; sy = iy; sz = iz; xs = 0; }
+    else if (grnd) { d = opp<L>(i); sy = iy; sz = iz; xs = 0; }
+    else if (D3 && yOut && zOut) { d = mirYZ<L>(i); sy = iy; sz = iz; xs = -cx; }
+    else if (yOut) { d = mirY<L>(i); sy = iy; sz = D3 ? min(max(iz - cz, 0), NZ - 1) : 0; xs = -cx; }
+    else if (D3 && zOut) {
+      if (P.sideMode == 1) { d = i; sy = iy - cy; sz = (iz - cz + NZ) % NZ; }
+      else { d = mirZ<L>(i); sy = iy - cy; sz = iz; }
+      xs = -cx;
+    } else { d = i; sy = iy - cy; sz = iz - cz; xs = -cx; }
+    const T* base = fin + (int64_t)d * n + ((int64_t)sy + (int64_t)NY * sz) * NX;
+    float va, vb = 0.0f;
+    if constexpr (CPT == 2) {
+      S::template ld2_dev<false>(base + (x0 + xs), L::w(i), va, vb);
+      if (cx < 0 && !grnd && x0 + 2 == NX) vb = va;  // outlet: x = nx clamps to nx - 1
+    } else {
+      const int xl = min(x0 + xs, NX - 1);
+      va = S::ld_dev(base, xl, L::w(i));
+    }
+    if (grnd && cx != 0) {  // moving belt, wgslLbm.ts:159-162
+      const float m = (6.0f * L::w(i) * cx) * P.uGround;
+      va += beltA * m;
+      vb += beltB * m;
+    }
+    if (cx > 0 && x0 == 0) va = inlet_dev<L>(i, U);  // inlet, wgslLbm.ts:146 (checked first there)
+    fa[i] = va;
+    fb[i] = vb;
   }
-  const uint32_t row = P.divx.div((uint32_t)c0);
-  const float x0 = (float)((uint32_t)c0 - row * (uint32_t)P.nx);
-  collide<L, false>(fa, x0, false, false, P);
-  collide<L, false>(fb, x0 + 1.0f, false, false, P);
+  collide<L>(fa, (float)x0, gwall, gwall, P);
+  if constexpr (CPT == 2) collide<L>(fb, (float)(x0 + 1), gwall, gwall, P);
 #pragma unroll
-  for (int i = 0; i < L::Q; i++) S::st2(fout, (int64_t)i * n + c0, L::w(i), fa[i], fb[i]);
+  for (int i = 0; i < Q; i++) {
+    T* o = fout + (int64_t)i * n + c0;
+    if constexpr (CPT == 2) S::st2_dev(o, L::w(i), fa[i], fb[i]);
+    else S::st_dev(o, 0, L::w(i), fa[i]);
+  }
 }
 
-// Boundary kernel: one thread per listed x-pair, the reference's full branch logic per cell.
+// Wall kernel: one thread per listed cell (cells next to the body), the reference's full branch
+// logic with precomputed solid masks.
 template <class L, class S>
-__global__ __launch_bounds__(BLOCK) void k_boundary(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                                    const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sdf,
-                                                    const uint8_t* __restrict__ kind, const uint32_t* __restrict__ list,
-                                                    int64_t count, KParams P, KWheels W) {
+__global__ __launch_bounds__(BLOCK) void k_wall(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sdf,
+                                                const uint8_t* __restrict__ kind, const uint32_t* __restrict__ list,
+                                                const uint32_t* __restrict__ masks, int64_t count, KParams P, KWheels W) {
   const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (t >= count) return;
-  const int64_t c0 = 2 * (int64_t)list[t];
-  float fa[L::Q], fb[L::Q];
-  update_general<L, S>(fin, flags, sdf, kind, P, W, c0, fa);
-  if (c0 + 1 < P.n) {
-    update_general<L, S>(fin, flags, sdf, kind, P, W, c0 + 1, fb);
+  const int64_t c = list[t];
+  int ix, iy, iz;
+  coords(c, P, ix, iy, iz);
+  float f[L::Q];
+  update_wall<L, S>(fin, flags, sdf, kind[c], masks[t], P, W, c, ix, iy, iz, f);
 #pragma unroll
-    for (int i = 0; i < L::Q; i++) S::st2(fout, (int64_t)i * P.n + c0, L::w(i), fa[i], fb[i]);
-  } else {  // odd cell count (FP32 storage only): the last cell has no partner
-#pragma unroll
-    for (int i = 0; i < L::Q; i++) fout[(int64_t)i * P.n + c0] = S::enc(fa[i], L::w(i));
-  }
+  for (int i = 0; i < L::Q; i++) S::st_dev(fout, (int64_t)i * P.n + c, L::w(i), f[i]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -183,11 +276,8 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
   double F0 = 0, F1 = 0, F2 = 0, B0 = 0, B1 = 0, B2 = 0, B3 = 0;
   float amax = 0.0f;
   if (idx < n) {
-    const uint32_t row = P.divx.div((uint32_t)idx);
-    const int ix = (int)((uint32_t)idx - row * (uint32_t)P.nx);
-    const uint32_t zz = P.divy.div(row);
-    const int iy = (int)(row - zz * (uint32_t)P.ny);
-    const int iz = (int)zz;
+    int ix, iy, iz;
+    coords(idx, P, ix, iy, iz);
     const int NX = P.nx, NY = P.ny, NZ = P.nz;
     auto safe = [&](int x, int y, int z) -> int64_t {
       if (x < 0 || y < 0 || z < 0 || x >= NX || y >= NY || z >= NZ) return -1;
@@ -199,19 +289,21 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
     if (!(kd & KIND_SOLID)) {
       float f[Q];
 #pragma unroll
-      for (int i = 0; i < Q; i++) f[i] = S::ld(fin, (int64_t)i * n + idx, L::w(i));
-      float rho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+      for (int i = 0; i < Q; i++) f[i] = S::ld_dev(fin, (int64_t)i * n + idx, L::w(i));
+      float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
 #pragma unroll
       for (int i = 0; i < Q; i++) {
-        rho += f[i];
+        drho += f[i];
         if (L::c(i, 0)) jx += L::c(i, 0) * f[i];
         if (L::c(i, 1)) jy += L::c(i, 1) * f[i];
         if (L::c(i, 2)) jz += L::c(i, 2) * f[i];
       }
+      const float rho = 1.0f + drho;
       const float ux = jx / rho, uy = jy / rho, uz = DIM == 3 ? jz / rho : 0.0f;
       out = make_float4(ux, uy, uz, rho);
       float Fx = 0.f, Fy = 0.f, Fz = 0.f, mvSum = 0.f, mvW = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
-      if (kd & KIND_BOUNDARY) {
+      if (kd & KIND_WALL) {
+        // link sums use f + f_r - 2 w = d + d_r (deviations)
 #pragma unroll
         for (int i = 1; i < Q; i++) {
           const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
@@ -221,7 +313,7 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
           float fr = f[i];
           if (q < 0.5f) {
             const int64_t bk = safe(ix - cx, iy - cy, iz - cz);
-            if (bk >= 0 && flags[bk] == 0u) fr = 2.0f * q * f[i] + (1.0f - 2.0f * q) * S::ld(fin, (int64_t)i * n + bk, L::w(i));
+            if (bk >= 0 && flags[bk] == 0u) fr = 2.0f * q * f[i] + (1.0f - 2.0f * q) * S::ld_dev(fin, (int64_t)i * n + bk, L::w(i));
             else q = 0.5f;
           } else if (q > 0.5f) {
             fr = (0.5f / q) * f[i] + (1.0f - 0.5f / q) * f[opp<L>(i)];
@@ -234,7 +326,7 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
             fr += mv; mvSum += mv; mvW += L::w(i);
             Cx += cx * L::w(i); Cy += cy * L::w(i); Cz += cz * L::w(i);
           }
-          const float m = f[i] + fr - 2.0f * L::w(i);
+          const float m = f[i] + fr;
           Fx += cx * m; Fy += cy * m; Fz += cz * m;
         }
         if (mvW > 0.0f) {
@@ -261,8 +353,8 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
         if (s >= 0 && flags[s] == 0u) {
           float r = 0.f;
 #pragma unroll
-          for (int i = 0; i < Q; i++) r += S::ld(fin, (int64_t)i * n + s, L::w(i));
-          rs += r; rn += 1.0f;
+          for (int i = 0; i < Q; i++) r += S::ld_dev(fin, (int64_t)i * n + s, L::w(i));
+          rs += 1.0f + r; rn += 1.0f;
         }
       }
       if (rn > 0.0f) out.w = rs / rn;
@@ -272,8 +364,9 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
       meanB[idx] = 0.0f;
       B3 = 1.0;
     }
-    vel[idx] = make_ushort4((unsigned short)f2h(out.x), (unsigned short)f2h(out.y), (unsigned short)f2h(out.z),
-                            (unsigned short)f2h(out.w));
+    const uint32_t lo = pack_h2(out.x, out.y), hi = pack_h2(out.z, out.w);
+    vel[idx] = make_ushort4((unsigned short)(lo & 0xffffu), (unsigned short)(lo >> 16), (unsigned short)(hi & 0xffffu),
+                            (unsigned short)(hi >> 16));
   }
   // block reduction: waves via cross-lane shuffles, then LDS across the 4 waves
   F0 = wave_sum(F0); F1 = wave_sum(F1); F2 = wave_sum(F2);
@@ -368,6 +461,8 @@ static void dispatch(int q, int prec, F&& fn) {
   }
 }
 
+int cells_per_thread(const KParams& P) { return (P.nx % 2 == 0) ? 2 : 1; }
+
 void launch_init(const LaunchCtx& c, void* f) {
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
@@ -375,27 +470,41 @@ void launch_init(const LaunchCtx& c, void* f) {
   });
 }
 
-void launch_classify(const LaunchCtx& c, uint32_t* list, unsigned long long* counts) {
-  if (c.P.q == 9) hipLaunchKernelGGL((k_classify<D2Q9>), dim3(blocks_for(c.P.n)), dim3(BLOCK), 0, c.stream, c.flags, c.kind, c.P);
-  else hipLaunchKernelGGL((k_classify<D3Q19>), dim3(blocks_for(c.P.n)), dim3(BLOCK), 0, c.stream, c.flags, c.kind, c.P);
-  const int64_t npairs = (c.P.n + 1) / 2;  // an odd last cell forms a pair with the zero padding
-  hipLaunchKernelGGL(k_collect, dim3(blocks_for(npairs)), dim3(BLOCK), 0, c.stream, (const uint16_t*)c.kind, npairs, list, counts);
+void launch_classify(const LaunchCtx& c, uint32_t* list, uint32_t* masks, unsigned long long* counts) {
+  const int cpt = cells_per_thread(c.P);
+  const int64_t ngroups = (c.P.n + cpt - 1) / cpt;
+  auto go = [&](auto l) {
+    using L = decltype(l);
+    hipLaunchKernelGGL((k_classify<L>), dim3(blocks_for(c.P.n)), dim3(BLOCK), 0, c.stream, c.flags, c.kind, c.P);
+    if (cpt == 2)
+      hipLaunchKernelGGL((k_collect<L, 2>), dim3(blocks_for(ngroups)), dim3(BLOCK), 0, c.stream, c.kind, c.flags, list, masks, counts, c.P);
+    else
+      hipLaunchKernelGGL((k_collect<L, 1>), dim3(blocks_for(ngroups)), dim3(BLOCK), 0, c.stream, c.kind, c.flags, list, masks, counts, c.P);
+  };
+  if (c.P.q == 9) go(D2Q9{});
+  else go(D3Q19{});
 }
 
 void launch_bulk(const LaunchCtx& c, const void* fin, void* fout) {
+  const int cpt = cells_per_thread(c.P);
+  const unsigned rows = (unsigned)((int64_t)c.P.ny * c.P.nz);
+  const dim3 grid((rows + 3) / 4, (unsigned)((c.P.nx + 64 * cpt - 1) / (64 * cpt)));
+  const dim3 block(64, 4);
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
-    hipLaunchKernelGGL((k_bulk<L, S>), dim3(blocks_for(c.P.n / 2)), dim3(BLOCK), 0, c.stream,
-                       (const typename S::T*)fin, (typename S::T*)fout, (const uint16_t*)c.kind, c.P);
+    if (cpt == 2)
+      hipLaunchKernelGGL((k_bulk<L, S, 2>), grid, block, 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind, c.P);
+    else
+      hipLaunchKernelGGL((k_bulk<L, S, 1>), grid, block, 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind, c.P);
   });
 }
 
-void launch_boundary(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, int64_t count) {
+void launch_wall(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, const uint32_t* masks, int64_t count) {
   if (count <= 0) return;
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
-    hipLaunchKernelGGL((k_boundary<L, S>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, (const typename S::T*)fin,
-                       (typename S::T*)fout, c.flags, c.sdf, c.kind, list, count, c.P, *c.W);
+    hipLaunchKernelGGL((k_wall<L, S>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, (const typename S::T*)fin,
+                       (typename S::T*)fout, c.flags, c.sdf, c.kind, list, masks, count, c.P, *c.W);
   });
 }
 

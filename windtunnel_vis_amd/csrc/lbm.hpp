@@ -8,17 +8,23 @@
 //   flags        u32 per cell: 0 fluid, 1 solid, 1 | (k+1) << 8 solid of wheel k (types as in
 //                src/solver/SolverGPU.ts:341-344).
 //   sdf          u32 per cell: ~bits(distance) or 0 = unknown (SolverGPU.ts:312-322).
-//   kind         u8 per cell, derived from flags: 0 interior fluid (bulk kernel), 1 solid,
-//                2 fluid that needs the boundary path, | 4 when a tyre contact patch holds the
-//                belt still at this floor cell (wgslLbm.ts:276-287).
+//   kind         u8 per cell, derived from flags (see KIND_* below).
+//
+// Arithmetic is FP32 on deviations d_i = f_i - w_i: moments are formed from d (rho = 1 + sum d,
+// j = sum c d, second moment = sum c c d + delta/3), which keeps the round-off relative to the
+// (small) deviations instead of to f.
 #pragma once
 #include <hip/hip_runtime.h>
-#include <hip/hip_fp16.h>
 #include <stdint.h>
 
 namespace wt {
 
-enum : uint8_t { KIND_FLUID = 0, KIND_SOLID = 1, KIND_BOUNDARY = 2, KIND_BELT_HOLD = 4 };
+enum : uint8_t {
+  KIND_FLUID = 0,      // fluid cell whose plain pulls all come from fluid cells: bulk kernel
+  KIND_SOLID = 1,      // solid cell (rest state)
+  KIND_WALL = 2,       // fluid cell with at least one plain pull from a solid cell: wall kernel
+  KIND_BELT_HOLD = 4,  // floor cell where a tyre contact patch holds the belt still (wgslLbm.ts:276-287)
+};
 
 // ---------------------------------------------------------------------------------------------
 // Lattices (src/solver/lattice.ts:39-60). opp / mirror tables are found by search at compile
@@ -32,7 +38,6 @@ struct D2Q9 {
     return t[i][a];
   }
   __host__ __device__ static constexpr float w(int i) { return i == 0 ? 4.0f / 9.0f : (i < 5 ? 1.0f / 9.0f : 1.0f / 36.0f); }
-  __host__ __device__ static constexpr double wd(int i) { return i == 0 ? 4.0 / 9.0 : (i < 5 ? 1.0 / 9.0 : 1.0 / 36.0); }
 };
 
 struct D3Q19 {
@@ -45,7 +50,6 @@ struct D3Q19 {
     return t[i][a];
   }
   __host__ __device__ static constexpr float w(int i) { return i == 0 ? 1.0f / 3.0f : (i < 7 ? 1.0f / 18.0f : 1.0f / 36.0f); }
-  __host__ __device__ static constexpr double wd(int i) { return i == 0 ? 1.0 / 3.0 : (i < 7 ? 1.0 / 18.0 : 1.0 / 36.0); }
 };
 
 template <class L>
@@ -60,40 +64,46 @@ template <class L> __host__ __device__ constexpr int mirZ(int i) { return find_d
 template <class L> __host__ __device__ constexpr int mirYZ(int i) { return find_dir<L>(L::c(i, 0), -L::c(i, 1), -L::c(i, 2)); }
 
 // ---------------------------------------------------------------------------------------------
-// Population storage formats.
+// Population storage formats. `*_dev` accessors move deviations d = f - w.
 // ---------------------------------------------------------------------------------------------
 typedef uint32_t u32_a2 __attribute__((aligned(2)));
 typedef uint64_t u64_a4 __attribute__((aligned(4)));
-typedef float f32x2 __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
-__device__ inline float h2f(uint32_t h) { return __half2float(__ushort_as_half((unsigned short)h)); }
-__device__ inline uint32_t f2h(float f) { return (uint32_t)__half_as_ushort(__float2half_rn(f)); }
-__device__ inline float b2f(uint32_t b) { return __uint_as_float(b << 16); }
-// float -> bfloat16 bits, round to nearest even (NaN kept quiet).
-__device__ inline uint32_t f2b(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
+// packed conversions (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32 on gfx950; round to nearest even)
+__device__ inline uint32_t pack_h2(float a, float b) { f32x2 v = {a, b}; return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, f16x2)); }
+__device__ inline uint32_t pack_b2(float a, float b) { f32x2 v = {a, b}; return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2)); }
+__device__ inline void unpack_h2(uint32_t u, float& a, float& b) {
+  f32x2 v = __builtin_convertvector(__builtin_bit_cast(f16x2, u), f32x2);
+  a = v.x; b = v.y;
 }
+__device__ inline void unpack_b2(uint32_t u, float& a, float& b) {
+  a = __uint_as_float(u << 16);
+  b = __uint_as_float(u & 0xffff0000u);
+}
+__device__ inline float h2f(uint32_t h) { float a, b; unpack_h2(h, a, b); return a; }
+__device__ inline float b2f(uint32_t h) { return __uint_as_float(h << 16); }
 
 struct SF32 {
   using T = float;
   static constexpr int BYTES = 4;
   __device__ static float ld(const T* p, int64_t e, float) { return p[e]; }
+  __device__ static float ld_dev(const T* p, int64_t e, float w) { return p[e] - w; }
   __device__ static T enc(float f, float) { return f; }
-  __device__ static T rest(float w) { return w; }
-  // two consecutive cells: e is the element index of the first
+  __device__ static void st_dev(T* p, int64_t e, float w, float d) { p[e] = w + d; }
+  // two consecutive cells; FP32 pairs are only 4-byte aligned in general
   template <bool ALIGNED>
-  __device__ static void ld2(const T* p, int64_t e, float, float& a, float& b) {
-    // FP32 pairs are only 4-byte aligned in general (odd nx or odd n allowed)
-    const uint64_t v = *(const u64_a4*)(p + e);
-    a = __uint_as_float((uint32_t)v); b = __uint_as_float((uint32_t)(v >> 32));
+  __device__ static void ld2_dev(const T* p, float w, float& a, float& b) {
+    const uint64_t v = *(const u64_a4*)p;
+    a = __uint_as_float((uint32_t)v) - w;
+    b = __uint_as_float((uint32_t)(v >> 32)) - w;
   }
-  __device__ static void st2(T* p, int64_t e, float w, float a, float b) {
-    (void)w;
-    f32x2 v = {a, b};
-    __builtin_nontemporal_store(v, (f32x2*)(p + e));
+  __device__ static void st2_dev(T* p, float w, float a, float b) {
+    f32x2a4 v = {w + a, w + b};
+    __builtin_nontemporal_store(v, (f32x2a4*)p);
   }
 };
 
@@ -101,43 +111,38 @@ struct SF16 {
   using T = uint16_t;
   static constexpr int BYTES = 2;
   __device__ static float ld(const T* p, int64_t e, float w) { return h2f(p[e]) + w; }
-  __device__ static T enc(float f, float w) { return (T)f2h(f - w); }
-  __device__ static T enc_dev(float d) { return (T)f2h(d); }
-  __device__ static T rest(float) { return 0; }
+  __device__ static float ld_dev(const T* p, int64_t e, float) { return h2f(p[e]); }
+  __device__ static T enc(float f, float w) { return (T)(pack_h2(f - w, 0.0f) & 0xffffu); }
+  __device__ static T enc_dev(float d) { return (T)(pack_h2(d, 0.0f) & 0xffffu); }
+  __device__ static void st_dev(T* p, int64_t e, float, float d) { p[e] = enc_dev(d); }
   template <bool ALIGNED>
-  __device__ static void ld2(const T* p, int64_t e, float w, float& a, float& b) {
-    uint32_t v = ALIGNED ? *(const uint32_t*)(p + e) : *(const u32_a2*)(p + e);
-    a = h2f(v & 0xffffu) + w;
-    b = h2f(v >> 16) + w;
+  __device__ static void ld2_dev(const T* p, float, float& a, float& b) {
+    const uint32_t v = ALIGNED ? *(const uint32_t*)p : *(const u32_a2*)p;
+    unpack_h2(v, a, b);
   }
-  __device__ static void st2(T* p, int64_t e, float w, float a, float b) {
-    uint32_t v = f2h(a - w) | (f2h(b - w) << 16);
-    __builtin_nontemporal_store(v, (uint32_t*)(p + e));
-  }
+  __device__ static void st2_dev(T* p, float, float a, float b) { __builtin_nontemporal_store(pack_h2(a, b), (uint32_t*)p); }
 };
 
 struct SBF16 {
   using T = uint16_t;
   static constexpr int BYTES = 2;
   __device__ static float ld(const T* p, int64_t e, float w) { return b2f(p[e]) + w; }
-  __device__ static T enc(float f, float w) { return (T)f2b(f - w); }
-  __device__ static T enc_dev(float d) { return (T)f2b(d); }
-  __device__ static T rest(float) { return 0; }
+  __device__ static float ld_dev(const T* p, int64_t e, float) { return b2f(p[e]); }
+  __device__ static T enc(float f, float w) { return (T)(pack_b2(f - w, 0.0f) & 0xffffu); }
+  __device__ static T enc_dev(float d) { return (T)(pack_b2(d, 0.0f) & 0xffffu); }
+  __device__ static void st_dev(T* p, int64_t e, float, float d) { p[e] = enc_dev(d); }
   template <bool ALIGNED>
-  __device__ static void ld2(const T* p, int64_t e, float w, float& a, float& b) {
-    uint32_t v = ALIGNED ? *(const uint32_t*)(p + e) : *(const u32_a2*)(p + e);
-    a = b2f(v & 0xffffu) + w;
-    b = b2f(v >> 16) + w;
+  __device__ static void ld2_dev(const T* p, float, float& a, float& b) {
+    const uint32_t v = ALIGNED ? *(const uint32_t*)p : *(const u32_a2*)p;
+    unpack_b2(v, a, b);
   }
-  __device__ static void st2(T* p, int64_t e, float w, float a, float b) {
-    uint32_t v = f2b(a - w) | (f2b(b - w) << 16);
-    __builtin_nontemporal_store(v, (uint32_t*)(p + e));
-  }
+  __device__ static void st2_dev(T* p, float, float a, float b) { __builtin_nontemporal_store(pack_b2(a, b), (uint32_t*)p); }
 };
 
 // ---------------------------------------------------------------------------------------------
 // Kernel parameter block: the reference's 96-byte uniform (wgslLbm.ts:6-15) as derived by
-// SolverGPU.writeParams (SolverGPU.ts:160-187), plus the fast-division constants.
+// SolverGPU.writeParams (SolverGPU.ts:160-187), plus derived constants (reciprocals of the sponge
+// widths, the Smagorinsky factor 18 sqrt2 Cs^2) and fast-division constants.
 // ---------------------------------------------------------------------------------------------
 struct FastDiv {  // unsigned 32-bit division by an invariant d >= 1 (round-up multiplier method)
   uint32_t d, m, s;
@@ -162,6 +167,10 @@ struct KParams {
   float spongeStart, spongeNu, uGround, emaAlpha;
   float uRef, tauWall, spongeIn, contactH;
   int32_t refX, closure, pad0, pad1;
+  float invSponge;    // 1 / max(nx - spongeStart, 1)
+  float invSpongeIn;  // 1 / max(spongeIn, 1)
+  float smag;         // 18 sqrt(2) Cs^2
+  float pad2;
   FastDiv divx, divy;
 };
 
@@ -191,119 +200,140 @@ __device__ inline float link_q(const uint32_t* sdf, const KParams& P, int64_t fl
   return fminf(fmaxf(a / (a + b), 0.01f), 0.99f);
 }
 
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Inlet equilibrium deviation w_i (3 c_x U + 4.5 c_x^2 U^2 - 1.5 U^2) (rho_in = 1), wgslLbm.ts:146.
+template <class L>
+__device__ inline float inlet_dev(int i, float U) {
+  const float cx = (float)L::c(i, 0);
+  return L::w(i) * (3.0f * cx * U + (4.5f * cx * cx - 1.5f) * U * U);
+}
+
 // ---------------------------------------------------------------------------------------------
-// Collision: moments, non-equilibrium stress, Smagorinsky tau, regularized (or BGK) relaxation,
-// outlet sponge and inlet absorbing layer (wgslLbm.ts:177-240). The update is written per
-// opposite-direction pair (i, opp i): the even part (mass, stress) and the odd part (momentum)
-// are formed once per pair, which halves the per-direction arithmetic. Algebraically identical
-// to the reference's per-direction form.
-// `nut_tau` (when >= 0) replaces the Smagorinsky tau by a closure-supplied relaxation time.
+// Collision on deviations (wgslLbm.ts:177-240): moments, non-equilibrium stress, Smagorinsky tau,
+// regularized (or BGK) relaxation, outlet sponge and inlet absorbing layer. Moments and the output
+// are formed per opposite-direction pair (i, opp i): the even part (mass, stress) and the odd part
+// (momentum) once per pair. Algebraically identical to the reference's per-direction form.
 // ---------------------------------------------------------------------------------------------
-template <class L, bool MAYBE_WALL>
-__device__ __forceinline__ void collide(float (&f)[L::Q], float fxpos, bool wall, bool gwall, const KParams& P) {
+template <class L>
+__device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall, bool gwall, const KParams& P) {
   constexpr int Q = L::Q;
-  float rho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
-  float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+  constexpr bool D3 = L::DIM == 3;
+  // pair sums / differences
+  float s[Q], a[Q];
+  float drho = d[0];
 #pragma unroll
-  for (int i = 0; i < Q; i++) {
-    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-    rho += f[i];
-    if (cx) jx += cx * f[i];
-    if (cy) jy += cy * f[i];
-    if (cz) jz += cz * f[i];
-    if (cx) sxx += f[i];
-    if (cy) syy += f[i];
-    if (cz) szz += f[i];
-    if (cx * cy) sxy += (cx * cy) * f[i];
-    if (cx * cz) sxz += (cx * cz) * f[i];
-    if (cy * cz) syz += (cy * cz) * f[i];
+  for (int i = 1; i < Q; i++) {
+    const int o = opp<L>(i);
+    if (o < i) continue;
+    s[i] = d[i] + d[o];
+    a[i] = d[i] - d[o];
+    drho += s[i];
   }
-  const float irho = 1.0f / rho;
-  const float ux = jx * irho, uy = jy * irho, uz = (L::DIM == 3) ? jz * irho : 0.0f;
-  const float usq = ux * ux + uy * uy + uz * uz;
+  float jx = 0.f, jy = 0.f, jz = 0.f, sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+#pragma unroll
+  for (int i = 1; i < Q; i++) {
+    const int o = opp<L>(i);
+    if (o < i) continue;
+    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
+    if (cx > 0) jx += a[i]; else if (cx < 0) jx -= a[i];
+    if (cy > 0) jy += a[i]; else if (cy < 0) jy -= a[i];
+    if (cz > 0) jz += a[i]; else if (cz < 0) jz -= a[i];
+    if (cx) sxx += s[i];
+    if (cy) syy += s[i];
+    if (cz) szz += s[i];
+    if (cx * cy > 0) sxy += s[i]; else if (cx * cy < 0) sxy -= s[i];
+    if (cx * cz > 0) sxz += s[i]; else if (cx * cz < 0) sxz -= s[i];
+    if (cy * cz > 0) syz += s[i]; else if (cy * cz < 0) syz -= s[i];
+  }
+  const float rho = 1.0f + drho;
+  const float irho = fast_rcp(rho);
+  const float ux = jx * irho, uy = jy * irho, uz = D3 ? jz * irho : 0.0f;
   const float third = 1.0f / 3.0f;
-  const float pxx = sxx - rho * (ux * ux + third);
-  const float pyy = syy - rho * (uy * uy + third);
-  const float pzz = (L::DIM == 3) ? szz - rho * (uz * uz + third) : 0.0f;
-  const float pxy = sxy - rho * ux * uy;
-  const float pxz = (L::DIM == 3) ? sxz - rho * ux * uz : 0.0f;
-  const float pyz = (L::DIM == 3) ? syz - rho * uy * uz : 0.0f;
-  const float piN = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
-  const float sp = fminf(fmaxf((fxpos - P.spongeStart) / fmaxf((float)P.nx - P.spongeStart, 1.0f), 0.0f), 1.0f);
+  const float dr3 = drho * third;
+  // Pi^neq = sum c c f - rho (u u + delta / 3) = S_dev - j u - drho delta / 3
+  const float pxx = sxx - jx * ux - dr3;
+  const float pyy = syy - jy * uy - dr3;
+  const float pzz = D3 ? szz - jz * uz - dr3 : 0.0f;
+  const float pxy = sxy - jx * uy;
+  const float pxz = D3 ? sxz - jx * uz : 0.0f;
+  const float pyz = D3 ? syz - jy * uz : 0.0f;
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float sp = __builtin_amdgcn_fmed3f((xpos - P.spongeStart) * P.invSponge, 0.0f, 1.0f);
   float t0 = P.tau0 + 3.0f * P.spongeNu * sp * sp;
-  if (MAYBE_WALL && gwall) t0 = fmaxf(t0, P.tauWall);
-  const float tau = 0.5f * (t0 + sqrtf(t0 * t0 + 25.4558441f * P.csmag2 * piN * irho));
-  const float om1 = 1.0f - 1.0f / tau;
-  const float k1 = om1 * 4.5f;
-  const bool useBGK = (MAYBE_WALL && wall) || P.collision == 1;
-  const float si = fminf(fmaxf(1.0f - fxpos / fmaxf(P.spongeIn, 1.0f), 0.0f), 1.0f);
+  if (gwall) t0 = fmaxf(t0, P.tauWall);
+  float tau = t0;
+  if (P.smag > 0.0f) {
+    const float piN = __builtin_sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+    tau = 0.5f * (t0 + __builtin_sqrtf(t0 * t0 + P.smag * piN * irho));
+  }
+  const float omega = fast_rcp(tau);
+  const float om1 = 1.0f - omega;
+  const float si = __builtin_amdgcn_fmed3f(1.0f - xpos * P.invSpongeIn, 0.0f, 1.0f);
   const float sIn = P.spongeIn > 0.0f ? 0.08f * si * si : 0.0f;
-  const float trP = (pxx + pyy + pzz) * third;
-  const float U = P.uin;
-  const float base_e = rho * (1.0f - 1.5f * usq);           // even part of feq / w
-  const float base_r = base_e - k1 * trP;                    // regularized even part / w (without c-terms)
+  const float rho45 = 4.5f * rho;
   float out[Q];
-  if (!useBGK) {
-    out[0] = L::w(0) * base_r;
+  if (!(wall || P.collision == 1)) {
+    // regularized: f_i - w_i = w_i (E + 4.5 rho cu^2 + k1 Q_i:Pi) +- 3 w_i c_i.j
+    const float k1 = 4.5f * om1;
+    const float trP = (pxx + pyy + pzz) * third;
+    const float E = drho - 1.5f * rho * usq - k1 * trP;
+    out[0] = L::w(0) * E;
 #pragma unroll
     for (int i = 1; i < Q; i++) {
       const int o = opp<L>(i);
       if (o < i) continue;
       const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
       const float cu = cx * ux + cy * uy + cz * uz;
+      const float cj = cx * jx + cy * jy + cz * jz;
       const float qp = (float)(cx * cx) * pxx + (float)(cy * cy) * pyy + (float)(cz * cz) * pzz +
                        (float)(2 * cx * cy) * pxy + (float)(2 * cx * cz) * pxz + (float)(2 * cy * cz) * pyz;
-      const float ev = L::w(i) * (base_r + 4.5f * rho * cu * cu + k1 * qp);
-      const float od = (3.0f * L::w(i)) * rho * cu;
+      const float ev = L::w(i) * (E + rho45 * cu * cu + k1 * qp);
+      const float od = (3.0f * L::w(i)) * cj;
       out[i] = ev + od;
       out[o] = ev - od;
     }
   } else {
-    out[0] = om1 * f[0] + (1.0f - om1) * (L::w(0) * base_e);
+    // BGK: f_i - w_i = omega Deq_i + (1 - omega) d_i
+    const float E = drho - 1.5f * rho * usq;
+    out[0] = omega * (L::w(0) * E) + om1 * d[0];
 #pragma unroll
     for (int i = 1; i < Q; i++) {
       const int o = opp<L>(i);
       if (o < i) continue;
       const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
       const float cu = cx * ux + cy * uy + cz * uz;
-      const float ev = L::w(i) * (base_e + 4.5f * rho * cu * cu);
-      const float od = (3.0f * L::w(i)) * rho * cu;
-      out[i] = (ev + od) + om1 * (f[i] - (ev + od));
-      out[o] = (ev - od) + om1 * (f[o] - (ev - od));
+      const float cj = cx * jx + cy * jy + cz * jz;
+      const float ev = L::w(i) * (E + rho45 * cu * cu);
+      const float od = (3.0f * L::w(i)) * cj;
+      out[i] = omega * (ev + od) + om1 * d[i];
+      out[o] = omega * (ev - od) + om1 * d[o];
     }
   }
-  if (sIn > 0.0f) {
+  if (sIn > 0.0f) {  // inlet absorbing layer: mix toward feq(1, U), wgslLbm.ts:220, :238
+    const float U = P.uin;
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
-      const int cx = L::c(i, 0);
-      const float tgt = L::w(i) * (1.0f + 3.0f * cx * U + 4.5f * (cx * cx) * U * U - 1.5f * U * U);
-      out[i] = out[i] * (1.0f - sIn) + tgt * sIn;
-    }
+    for (int i = 0; i < Q; i++) out[i] = out[i] + sIn * (inlet_dev<L>(i, U) - out[i]);
   }
 #pragma unroll
-  for (int i = 0; i < Q; i++) f[i] = out[i];
+  for (int i = 0; i < Q; i++) d[i] = out[i];
 }
 
 // ---------------------------------------------------------------------------------------------
-// General per-cell pull + boundaries (wgslLbm.ts:139-175, :262-291), exactly the reference's
-// branch order. Returns false for a solid cell (out = w).
+// General per-cell pull + boundaries (wgslLbm.ts:139-175, :262-291), the reference's branch order,
+// for cells next to the body. `smask` bit i is set when the plain pull of direction i comes from a
+// solid cell (precomputed from flags). Produces post-collision deviations.
 // ---------------------------------------------------------------------------------------------
 template <class L, class S>
-__device__ inline void update_general(const typename S::T* __restrict__ fin, const uint32_t* __restrict__ flags,
-                                      const uint32_t* __restrict__ sdf, const uint8_t* __restrict__ kind,
-                                      const KParams& P, const KWheels& W, int64_t idx, float (&f)[L::Q]) {
+__device__ inline void update_wall(const typename S::T* __restrict__ fin, const uint32_t* __restrict__ flags,
+                                   const uint32_t* __restrict__ sdf, uint8_t kd, uint32_t smask, const KParams& P,
+                                   const KWheels& W, int64_t idx, int ix, int iy, int iz, float (&f)[L::Q]) {
   constexpr int Q = L::Q, DIM = L::DIM;
   const int64_t n = P.n;
   const int NX = P.nx, NY = P.ny, NZ = P.nz;
-  const uint32_t row = P.divx.div((uint32_t)idx);
-  const int ix = (int)((uint32_t)idx - row * (uint32_t)NX);
-  const uint32_t zz = P.divy.div(row);
-  const int iy = (int)(row - zz * (uint32_t)NY);
-  const int iz = (int)zz;
-  const uint8_t kd = kind[idx];
   if (kd & KIND_SOLID) {
 #pragma unroll
-    for (int i = 0; i < Q; i++) f[i] = L::w(i);
+    for (int i = 0; i < Q; i++) f[i] = 0.0f;
     return;
   }
   auto cell = [&](int x, int y, int z) -> int64_t { return (int64_t)x + (int64_t)NX * ((int64_t)y + (int64_t)NY * z); };
@@ -311,8 +341,8 @@ __device__ inline void update_general(const typename S::T* __restrict__ fin, con
   bool wall = false, gwall = false;
   float mvSum = 0.f, mvW = 0.f;
   uint32_t wm = 0u;
-  const float belt = (P.ground == 2 && !(iy == 0 && (kd & KIND_BELT_HOLD))) ? 1.0f : 0.0f;
-  f[0] = S::ld(fin, idx, L::w(0));
+  const float belt = (P.ground == 2 && !(kd & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
+  f[0] = S::ld_dev(fin, idx, L::w(0));
 #pragma unroll
   for (int i = 1; i < Q; i++) {
     const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
@@ -323,38 +353,37 @@ __device__ inline void update_general(const typename S::T* __restrict__ fin, con
     const bool zOut = DIM == 3 ? (cz > 0 ? sz < 0 : (cz < 0 ? sz >= NZ : false)) : false;
     float v;
     if (cx > 0 && sx < 0) {
-      const float cu = (float)cx * U;
-      v = w * P.rhoIn * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * U * U);
+      v = inlet_dev<L>(i, U);
     } else if (cy > 0 && yOut && P.ground != 0) {
       wall = true; gwall = true;
-      v = S::ld(fin, (int64_t)opp<L>(i) * n + idx, L::w(opp<L>(i)));
+      v = S::ld_dev(fin, (int64_t)opp<L>(i) * n + idx, w);
       if (cx != 0) v += belt * ((6.0f * w * cx) * P.uGround);
     } else {
       const int sxc = min(max(sx, 0), NX - 1);
       if (DIM == 3 && yOut && zOut) {
-        v = S::ld(fin, (int64_t)mirYZ<L>(i) * n + cell(sxc, iy, iz), L::w(mirYZ<L>(i)));
+        v = S::ld_dev(fin, (int64_t)mirYZ<L>(i) * n + cell(sxc, iy, iz), w);
       } else if (yOut) {
-        v = S::ld(fin, (int64_t)mirY<L>(i) * n + cell(sxc, iy, DIM == 3 ? min(max(sz, 0), NZ - 1) : 0), L::w(mirY<L>(i)));
+        v = S::ld_dev(fin, (int64_t)mirY<L>(i) * n + cell(sxc, iy, DIM == 3 ? min(max(sz, 0), NZ - 1) : 0), w);
       } else if (DIM == 3 && zOut) {
-        if (P.sideMode == 1) v = S::ld(fin, (int64_t)i * n + cell(sxc, sy, (sz + NZ) % NZ), w);
-        else v = S::ld(fin, (int64_t)mirZ<L>(i) * n + cell(sxc, sy, iz), L::w(mirZ<L>(i)));
+        if (P.sideMode == 1) v = S::ld_dev(fin, (int64_t)i * n + cell(sxc, sy, (sz + NZ) % NZ), w);
+        else v = S::ld_dev(fin, (int64_t)mirZ<L>(i) * n + cell(sxc, sy, iz), w);
       } else {
         const int64_t sc = cell(sxc, sy, sz);
-        const uint32_t fs = flags[sc];
-        if (fs != 0u) {
+        if (smask & (1u << i)) {
           wall = true;
-          const float fb = S::ld(fin, (int64_t)opp<L>(i) * n + idx, L::w(opp<L>(i)));
+          const float fb = S::ld_dev(fin, (int64_t)opp<L>(i) * n + idx, w);
           float q = link_q(sdf, P, idx, sc);
           v = fb;
           if (q < 0.5f) {
             const int ax = ix + cx, ay = iy + cy, az = DIM == 3 ? iz + cz : 0;
             if (ax >= 0 && ay >= 0 && az >= 0 && ax < NX && ay < NY && az < NZ && flags[cell(ax, ay, az)] == 0u)
-              v = 2.0f * q * fb + (1.0f - 2.0f * q) * S::ld(fin, (int64_t)opp<L>(i) * n + cell(ax, ay, az), L::w(opp<L>(i)));
+              v = 2.0f * q * fb + (1.0f - 2.0f * q) * S::ld_dev(fin, (int64_t)opp<L>(i) * n + cell(ax, ay, az), w);
             else
               q = 0.5f;
           } else if (q > 0.5f) {
-            v = (0.5f / q) * fb + (1.0f - 0.5f / q) * S::ld(fin, (int64_t)i * n + idx, w);
+            v = (0.5f / q) * fb + (1.0f - 0.5f / q) * S::ld_dev(fin, (int64_t)i * n + idx, w);
           }
+          const uint32_t fs = flags[sc];
           if ((fs >> 8) != 0u) {
             const float3 p = make_float3(0.5f * (float)(ix + sxc) + 0.5f, 0.5f * (float)(iy + sy) + 0.5f,
                                          0.5f * (float)(iz + sz) + 0.5f);
@@ -363,19 +392,19 @@ __device__ inline void update_general(const typename S::T* __restrict__ fin, con
             v += mv; mvSum += mv; mvW += w; wm |= 1u << i;
           }
         } else {
-          v = S::ld(fin, (int64_t)i * n + sc, w);
+          v = S::ld_dev(fin, (int64_t)i * n + sc, w);
         }
       }
     }
     f[i] = v;
   }
-  if (mvW > 0.0f) {
+  if (mvW > 0.0f) {  // wheel mass fix, wgslLbm.ts:289-291
     const float corr = mvSum / mvW;
 #pragma unroll
     for (int i = 1; i < Q; i++)
       if (wm & (1u << i)) f[i] -= L::w(i) * corr;
   }
-  collide<L, true>(f, (float)ix, wall, gwall, P);
+  collide<L>(f, (float)ix, wall, gwall, P);
 }
 
 }  // namespace wt

@@ -46,7 +46,8 @@ struct wt_solver {
   uint32_t* flags = nullptr;
   uint32_t* sdf = nullptr;
   uint8_t* kind = nullptr;
-  uint32_t* blist = nullptr;
+  uint32_t* wlist = nullptr;   // cells handled by the wall kernel
+  uint32_t* wmask = nullptr;   // their solid masks
   unsigned long long* counts = nullptr;
   ushort4* vel = nullptr;
   float4* meanA = nullptr;
@@ -54,14 +55,14 @@ struct wt_solver {
   double* partials = nullptr;
   double* result = nullptr;
   double* h_result = nullptr;  // pinned
-  int64_t nboundary = 0, nfast = 0, nsolid = 0;
+  int64_t nwall = 0, nfast = 0, nsolid = 0;  // cells: wall kernel, bulk fluid, bulk solid
   int parity = 0;
   bool pending_init = true;
   int64_t step = 0;
   int64_t bytes = 0;
   // profiling
   bool prof = false;
-  std::vector<hipEvent_t> ev;   // pairs: bulk start/stop, boundary start/stop per step
+  std::vector<hipEvent_t> ev;   // per step: bulk start, bulk stop (= wall start), -, wall stop
   size_t ev_used = 0;
   double acc_bulk = 0, acc_bnd = 0;
   int64_t acc_launches = 0;
@@ -105,6 +106,9 @@ void derive(wt_solver* s) {
   P.contactH = 2.0f;
   P.interp = p.interp ? 1 : 0;
   P.closure = p.closure;
+  P.invSponge = (float)(1.0 / std::max((double)s->dims.nx - p.sponge_start * s->dims.nx, 1.0));
+  P.invSpongeIn = (float)(1.0 / std::max(p.sponge_in * s->dims.nx, 1.0));
+  P.smag = (float)(18.0 * std::sqrt(2.0) * p.cs * p.cs);
   P.divx.init((uint32_t)s->dims.nx);
   P.divy.init((uint32_t)s->dims.ny);
 }
@@ -133,12 +137,12 @@ int run_init(wt_solver* s) {
 int reclassify(wt_solver* s) {
   HIP_TRY(hipMemsetAsync(s->counts, 0, 4 * sizeof(unsigned long long), s->stream));
   LaunchCtx c = s->ctx();
-  launch_classify(c, s->blist, s->counts);
+  launch_classify(c, s->wlist, s->wmask, s->counts);
   HIP_TRY(hipGetLastError());
   unsigned long long h[4];
   HIP_TRY(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
-  s->nboundary = (int64_t)h[0];
+  s->nwall = (int64_t)h[0];
   s->nfast = (int64_t)h[1];
   s->nsolid = (int64_t)h[2];
   return WT_OK;
@@ -164,12 +168,12 @@ int do_steps(wt_solver* s, int steps) {
       HIP_TRY(hipEventRecord(e[0], s->stream));
       launch_bulk(c, fin, fout);
       HIP_TRY(hipEventRecord(e[1], s->stream));
-      launch_boundary(c, fin, fout, s->blist, s->nboundary);
+      launch_wall(c, fin, fout, s->wlist, s->wmask, s->nwall);
       HIP_TRY(hipEventRecord(e[3], s->stream));
       e[2] = e[1];
     } else {
       launch_bulk(c, fin, fout);
-      launch_boundary(c, fin, fout, s->blist, s->nboundary);
+      launch_wall(c, fin, fout, s->wlist, s->wmask, s->nwall);
     }
     s->parity ^= 1;
   }
@@ -233,7 +237,7 @@ int wt_create(const wt_dims* dims, const wt_params* params, wt_solver** out) {
   };
   // f[] gets one extra word of padding so pair loads at the very last cell stay in bounds
   bool ok = alloc(&s->f[0], fbytes + 64) && alloc(&s->f[1], fbytes + 64) && alloc((void**)&s->flags, 4 * n) &&
-            alloc((void**)&s->sdf, 4 * n) && alloc((void**)&s->kind, n + 64) && alloc((void**)&s->blist, 4 * (n / 2 + 1)) &&
+            alloc((void**)&s->sdf, 4 * n) && alloc((void**)&s->kind, n + 64) && alloc((void**)&s->wlist, 4 * (n + 1)) && alloc((void**)&s->wmask, 4 * (n + 1)) &&
             alloc((void**)&s->counts, 64) && alloc((void**)&s->vel, 8 * n) && alloc((void**)&s->meanA, 16 * n) &&
             alloc((void**)&s->meanB, 4 * n) && alloc((void**)&s->partials, 64 * nblocks) && alloc((void**)&s->result, 64);
   if (!ok) {
@@ -264,7 +268,7 @@ int wt_destroy(wt_solver* s) {
   if (!s) return WT_OK;
   if (s->stream) hipStreamSynchronize(s->stream);
   for (auto e : s->ev) hipEventDestroy(e);
-  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->blist, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
+  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (s->h_result) hipHostFree(s->h_result);
@@ -280,7 +284,7 @@ int wt_info_get(const wt_solver* s, wt_info* out) {
   out->q = s->q;
   out->precision = s->prec;
   out->step = s->step;
-  out->boundary_pairs = s->nboundary;
+  out->wall_cells = s->nwall;
   out->device_bytes = s->bytes;
   return WT_OK;
 }
@@ -500,8 +504,8 @@ int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* bo
 
 int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cells) {
   if (!s) return fail(WT_E_ARG, "null argument");
-  if (bulk_cells) *bulk_cells = 2 * s->nfast;
-  if (boundary_cells) *boundary_cells = 2 * s->nboundary;
+  if (bulk_cells) *bulk_cells = s->nfast;
+  if (boundary_cells) *boundary_cells = s->nwall;
   return WT_OK;
 }
 
