@@ -1,0 +1,19 @@
+// Drives the Node host on a small seeded case and prints populations statistics as JSON, for
+// tests/test_gpu_node_host.py to compare with the Python host and the oracle.
+import { SolverGPU } from '../../windtunnel_vis_amd/node/solver.mjs';
+
+async function main() {
+  const dims = { nx: 24, ny: 12, nz: 10 };
+  const s = new SolverGPU(dims, { u: 0.06, nu: 0.01, cs: 0.1, ground: 'moving', sides: 'freeslip', spongeNu: 0.04,
+    spongeStart: 0.8, spongeIn: 0.15, tauWall: 0.53, emaAlpha: 0.1, precision: process.argv[2] || 'f16' });
+  const flags = new Uint32Array(dims.nx * dims.ny * dims.nz);
+  for (let z = 3; z < 7; z++) for (let y = 4; y < 8; y++) for (let x = 9; x < 13; x++) flags[x + dims.nx * (y + dims.ny * z)] = 1;
+  s.uploadFlags(flags);
+  const st = await s.stepAndSample(100);
+  const f = await s.readPopulations();
+  let sum = 0;
+  for (const v of f) sum += v;
+  console.log(JSON.stringify({ stats: st, sum, first: Array.from(f.slice(0, 8)), stepCount: s.stepCount, precision: s.precision }));
+  s.destroy();
+}
+main().catch((e) => { console.error(e); process.exit(2); });

@@ -60,8 +60,9 @@ def build_node_addon(verbose=False):
     if not os.path.exists(src):
         return None
     out = os.path.join(LIB_DIR, "wt_addon.node")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", node_inc, "-I", os.path.join(HERE, "..", "include"),
-          src, "-o", out + ".tmp", "-L", LIB_DIR, "-lwt", "-Wl,-rpath,$ORIGIN", "-Wl,--unresolved-symbols=ignore-in-object-files"])
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-DNODE_GYP_MODULE_NAME=wt_addon", "-I", node_inc,
+          "-I", os.path.join(HERE, "..", "include"), src, "-o", out + ".tmp", "-L", LIB_DIR, "-lwt",
+          "-Wl,-rpath,$ORIGIN", "-Wl,--unresolved-symbols=ignore-in-object-files"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -1,0 +1,75 @@
+// Types of the Node host (solver.mjs): the reference's SolverParams / SolverStats / GridDims
+// (src/solver/types.ts) and the LbmSolver contract (src/backend/types.ts:7-18).
+export type GroundMode = 'freeslip' | 'noslip' | 'moving';
+
+export interface SolverParams {
+  u: number;
+  nu: number;
+  cs: number;
+  ground: GroundMode;
+  sides: 'freeslip' | 'periodic';
+  spongeNu: number;
+  spongeStart: number;
+  spongeIn?: number;
+  tauWall?: number;
+  precision?: 'f32' | 'f16' | 'bf16';
+  interp?: boolean;
+  collision?: 'regularized' | 'bgk';
+  closure?: 'smagorinsky' | 'mlp';
+  emaAlpha: number;
+}
+
+export interface SolverStats {
+  fx: number; fy: number; fz: number;
+  maxU: number;
+  rhoRef: number;
+  unstable: boolean;
+  step: number;
+}
+
+export interface GridDims { nx: number; ny: number; nz: number; }
+
+export interface WheelDef { c: number[]; axis: number[]; r: number; hw: number; }
+
+export interface VoxelInfo { frontal: number; solidCells: number; min: [number, number, number]; max: [number, number, number]; }
+
+export interface LbmSolver {
+  readonly dims: GridDims;
+  readonly n: number;
+  stepCount: number;
+  uploadFlags(flags: Uint32Array): void;
+  uploadSdf?(d: Float32Array): void;
+  stepAndSample(steps: number): Promise<SolverStats>;
+  destroy(): void;
+}
+
+export type SolverFactory = (dims: GridDims, p: SolverParams) => LbmSolver;
+
+export declare class SolverGPU implements LbmSolver {
+  constructor(dims: GridDims, params: SolverParams);
+  readonly dims: GridDims;
+  readonly n: number;
+  readonly is3D: boolean;
+  readonly precision: 'f32' | 'f16' | 'bf16';
+  readonly packed: boolean;
+  latest: SolverStats | null;
+  readonly stepCount: number;
+  readonly tau0: number;
+  setParams(p: Partial<SolverParams>): void;
+  reset(): void;
+  clearStats(): void;
+  encodeSteps(steps: number): void;
+  stepAndSample(steps: number): Promise<SolverStats>;
+  uploadFlags(flags: Uint32Array): void;
+  readFlags(): Promise<Uint32Array>;
+  uploadSdf(d: Float32Array): void;
+  setWheels(list: WheelDef[]): void;
+  readPopulations(): Promise<Float32Array>;
+  uploadPopulations(f: Float32Array): void;
+  voxelizeSdf(prims: Float32Array): VoxelInfo;
+  voxelizeMesh(mesh: { positions: ArrayLike<number>; indices: ArrayLike<number> }, matrix: ArrayLike<number>, wheels?: WheelDef[]): VoxelInfo;
+  destroy(): void;
+  static bytesFor(dims: GridDims): number;
+}
+
+export declare const solverFactory: SolverFactory;

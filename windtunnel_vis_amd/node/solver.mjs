@@ -1,0 +1,131 @@
+// Node host of the MI355X solver: the reference's SolverGPU surface (src/solver/SolverGPU.ts) and
+// its LbmSolver / SolverFactory contract (src/backend/types.ts:7-18), over the N-API addon that
+// binds libwt.so. Types: solver.d.ts.
+import { createRequire } from 'module';
+import { dirname, join } from 'path';
+import { fileURLToPath } from 'url';
+
+const require = createRequire(import.meta.url);
+const here = dirname(fileURLToPath(import.meta.url));
+const native = require(join(here, '..', '_lib', 'wt_addon.node'));
+
+const GROUND = { freeslip: 0, noslip: 1, moving: 2 };
+const SIDES = { freeslip: 0, periodic: 1 };
+const PRECISION = { f32: 0, f16: 1, bf16: 2 };
+const PRECISION_NAME = ['f32', 'f16', 'bf16'];
+const COLLISION = { regularized: 0, bgk: 1 };
+const CLOSURE = { smagorinsky: 0, mlp: 1 };
+
+/** D2Q9 / D3Q19 velocity sets (src/solver/lattice.ts:39-60). */
+export const D2Q9 = {
+  name: 'D2Q9', dim: 2, q: 9,
+  c: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0], [1, 1, 0], [-1, 1, 0], [-1, -1, 0], [1, -1, 0]],
+  w: [4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36],
+};
+export const D3Q19 = {
+  name: 'D3Q19', dim: 3, q: 19,
+  c: [[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0], [-1, -1, 0], [1, 0, 1],
+    [-1, 0, -1], [0, 1, 1], [0, -1, -1], [1, -1, 0], [-1, 1, 0], [1, 0, -1], [-1, 0, 1], [0, 1, -1], [0, -1, 1]],
+  w: [1 / 3, ...Array(6).fill(1 / 18), ...Array(12).fill(1 / 36)],
+};
+
+/** SolverParams (src/solver/types.ts:3-29) -> the addon's numeric form. Only present keys are sent. */
+function encodeParams(p) {
+  const o = {};
+  const num = ['u', 'nu', 'cs', 'spongeNu', 'spongeStart', 'spongeIn', 'tauWall', 'emaAlpha'];
+  for (const k of num) if (p[k] !== undefined && p[k] !== null) o[k] = p[k];
+  if (p.ground !== undefined) o.ground = GROUND[p.ground];
+  if (p.sides !== undefined) o.sides = SIDES[p.sides];
+  if (p.interp !== undefined) o.interp = p.interp === false ? 0 : 1;
+  if (p.collision !== undefined) o.collision = COLLISION[p.collision];
+  if (p.closure !== undefined) o.closure = CLOSURE[p.closure];
+  if (p.precision !== undefined) o.precision = PRECISION[p.precision];
+  for (const [k, v] of Object.entries(o)) if (v === undefined) throw new TypeError(`bad value for params.${k}`);
+  return o;
+}
+
+function wheelArray(list) {
+  const n = Math.min(list.length, 16);
+  const a = new Float32Array(8 * n);
+  for (let k = 0; k < n; k++) {
+    const w = list[k];
+    a.set([w.c[0], w.c[1], w.c[2], w.axis[0], w.axis[1], w.axis[2], w.r, w.hw], 8 * k);
+  }
+  return a;
+}
+
+/**
+ * GPU lattice Boltzmann solver (D2Q9 when nz == 1, D3Q19 otherwise): SolverGPU.ts:12-444.
+ * The constructor takes (dims, params) — the device is the process's HIP device.
+ */
+export class SolverGPU {
+  constructor(dims, params) {
+    this.dims = { nx: dims.nx, ny: dims.ny, nz: dims.nz };
+    this.is3D = dims.nz > 1;
+    this.lattice = this.is3D ? D3Q19 : D2Q9;
+    this.n = dims.nx * dims.ny * dims.nz;
+    this.p = { tauWall: 0.5, spongeIn: 0, interp: true, collision: 'regularized', precision: 'f32', ...params };
+    this.h = native.create(this.dims, encodeParams(this.p));
+    const info = native.info(this.h);
+    this.precision = PRECISION_NAME[info.precision];
+    /** populations stored as packed half-width deviation pairs */
+    this.packed = this.precision !== 'f32';
+    this.latest = null;
+    this.queue = Promise.resolve();
+  }
+
+  get stepCount() { return native.info(this.h).step; }
+  get params_() { return this.p; }
+  /** Current tau0 (molecular relaxation time), SolverGPU.ts:151. */
+  get tau0() { return 3 * this.p.nu + 0.5; }
+
+  setParams(p) {
+    Object.assign(this.p, p);
+    native.setParams(this.h, encodeParams(p));
+  }
+
+  reset() { native.reset(this.h); }
+  clearStats() { native.clearStats(this.h); }
+
+  /** Record `steps` steps (asynchronous on the GPU), SolverGPU.ts:204-222. */
+  encodeSteps(steps) { native.step(this.h, steps); }
+
+  /** Run `steps` steps, then read the force / statistics result (SolverGPU.ts:292-309). */
+  stepAndSample(steps) {
+    // one handle is not re-entrant: serialise the async work
+    const run = this.queue.then(() => native.stepAndSample(this.h, steps));
+    this.queue = run.then(() => undefined, () => undefined);
+    return run.then((s) => {
+      if (!this.latest || s.step >= this.latest.step) this.latest = s;
+      return s;
+    });
+  }
+
+  uploadFlags(flags) { native.uploadFlags(this.h, flags); }
+  async readFlags() { return native.readFlags(this.h); }
+  /** Distances to the wall (cells, < 0 = unknown) for interpolated bounce-back, SolverGPU.ts:312. */
+  uploadSdf(d) { native.uploadSdf(this.h, d); }
+  /** Rotating wheels in grid space (at most 16), SolverGPU.ts:325-335. */
+  setWheels(list) { native.setWheels(this.h, wheelArray(list)); }
+  /** Current populations, unpacked to full f_i (q * n floats). */
+  async readPopulations() { return native.readPopulations(this.h); }
+  uploadPopulations(f) { native.uploadPopulations(this.h, f); }
+
+  /** Analytic-SDF body (list of primitives, see geometry.mjs). */
+  voxelizeSdf(prims) { return native.voxelizeSdf(this.h, prims); }
+  /** VoxelizerGPU.voxelize(solver, mesh, matrix, wheels), VoxelizerGPU.ts:48-151. */
+  voxelizeMesh(mesh, matrix, wheels = []) {
+    return native.voxelizeMesh(this.h, Float32Array.from(mesh.positions), Uint32Array.from(mesh.indices),
+      Float32Array.from(matrix), wheelArray(wheels));
+  }
+
+  destroy() {
+    if (this.h) native.destroy(this.h);
+    this.h = null;
+  }
+
+  static bytesFor(dims) { return native.bytesFor(dims); }
+}
+
+/** SolverFactory (src/backend/types.ts:18). */
+export const solverFactory = (dims, p) => new SolverGPU(dims, p);

@@ -1,0 +1,513 @@
+// N-API addon binding libwt.so for the Node host (solver.mjs). Plain C N-API (node_api.h), no
+// extra packages. Each solver handle is a JS external; stepAndSample runs as async work off the
+// JS thread and resolves a Promise, like the reference's mapAsync-based stepAndSample
+// (src/solver/SolverGPU.ts:292-309).
+#include <node_api.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wt.h"
+
+namespace {
+
+#define NAPI_CALL(env, call)                                              \
+  do {                                                                    \
+    if ((call) != napi_ok) {                                              \
+      napi_throw_error(env, nullptr, "N-API call failed: " #call);        \
+      return nullptr;                                                     \
+    }                                                                     \
+  } while (0)
+
+napi_value throw_wt(napi_env env, int rc) {
+  std::string msg = std::string("libwt error ") + std::to_string(rc) + ": " + wt_last_error();
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+struct Handle {
+  wt_solver* s = nullptr;
+};
+
+void finalize_handle(napi_env, void* data, void*) {
+  auto* h = static_cast<Handle*>(data);
+  if (h->s) wt_destroy(h->s);
+  delete h;
+}
+
+Handle* get_handle(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "expected a solver handle");
+    return nullptr;
+  }
+  auto* h = static_cast<Handle*>(p);
+  if (!h->s) {
+    napi_throw_error(env, nullptr, "solver was destroyed");
+    return nullptr;
+  }
+  return h;
+}
+
+bool get_prop_num(napi_env env, napi_value obj, const char* key, double* out) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return false;
+  napi_value v;
+  napi_get_named_property(env, obj, key, &v);
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_boolean) {
+    bool b;
+    napi_get_value_bool(env, v, &b);
+    *out = b ? 1.0 : 0.0;
+    return true;
+  }
+  if (t != napi_number) return false;
+  napi_get_value_double(env, v, out);
+  return true;
+}
+
+// params object: numeric fields with the C names; enums already mapped to integers by solver.mjs.
+uint32_t parse_params(napi_env env, napi_value obj, wt_params* p) {
+  uint32_t mask = 0;
+  double v;
+  struct F { const char* k; uint32_t bit; int kind; void* dst; };
+  F fields[] = {
+      {"u", WT_P_U, 0, &p->u}, {"nu", WT_P_NU, 0, &p->nu}, {"cs", WT_P_CS, 0, &p->cs},
+      {"ground", WT_P_GROUND, 1, &p->ground}, {"sides", WT_P_SIDES, 1, &p->sides},
+      {"spongeNu", WT_P_SPONGE_NU, 0, &p->sponge_nu}, {"spongeStart", WT_P_SPONGE_START, 0, &p->sponge_start},
+      {"spongeIn", WT_P_SPONGE_IN, 0, &p->sponge_in}, {"tauWall", WT_P_TAU_WALL, 0, &p->tau_wall},
+      {"interp", WT_P_INTERP, 1, &p->interp}, {"collision", WT_P_COLLISION, 1, &p->collision},
+      {"closure", WT_P_CLOSURE, 1, &p->closure}, {"emaAlpha", WT_P_EMA_ALPHA, 0, &p->ema_alpha},
+  };
+  for (auto& f : fields) {
+    if (!get_prop_num(env, obj, f.k, &v)) continue;
+    if (f.kind == 0) *(double*)f.dst = v;
+    else *(int32_t*)f.dst = (int32_t)v;
+    mask |= f.bit;
+  }
+  if (get_prop_num(env, obj, "precision", &v)) p->precision = (int32_t)v;
+  return mask;
+}
+
+template <class T>
+bool typed_array(napi_env env, napi_value v, napi_typedarray_type want, T** data, size_t* len) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) return false;
+  napi_typedarray_type t;
+  void* d;
+  napi_value ab;
+  size_t off;
+  napi_get_typedarray_info(env, v, &t, len, &d, &ab, &off);
+  if (t != want) return false;
+  *data = static_cast<T*>(d);
+  return true;
+}
+
+napi_value make_typed(napi_env env, napi_typedarray_type t, size_t len, size_t elem, void** data) {
+  napi_value ab, out;
+  if (napi_create_arraybuffer(env, len * elem, data, &ab) != napi_ok) return nullptr;
+  if (napi_create_typedarray(env, t, len, ab, 0, &out) != napi_ok) return nullptr;
+  return out;
+}
+
+void set_num(napi_env env, napi_value obj, const char* k, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, obj, k, x);
+}
+
+napi_value stats_object(napi_env env, const wt_stats& st) {
+  napi_value o, b;
+  napi_create_object(env, &o);
+  set_num(env, o, "fx", st.fx);
+  set_num(env, o, "fy", st.fy);
+  set_num(env, o, "fz", st.fz);
+  set_num(env, o, "maxU", st.max_u);
+  set_num(env, o, "rhoRef", st.rho_ref);
+  napi_get_boolean(env, st.unstable != 0, &b);
+  napi_set_named_property(env, o, "unstable", b);
+  set_num(env, o, "step", (double)st.step);
+  return o;
+}
+
+napi_value voxel_object(napi_env env, const wt_voxel_info& vi) {
+  napi_value o, mn, mx;
+  napi_create_object(env, &o);
+  set_num(env, o, "frontal", (double)vi.frontal);
+  set_num(env, o, "solidCells", (double)vi.solid_cells);
+  napi_create_array_with_length(env, 3, &mn);
+  napi_create_array_with_length(env, 3, &mx);
+  for (uint32_t a = 0; a < 3; a++) {
+    napi_value x;
+    napi_create_int32(env, vi.min[a], &x);
+    napi_set_element(env, mn, a, x);
+    napi_create_int32(env, vi.max[a], &x);
+    napi_set_element(env, mx, a, x);
+  }
+  napi_set_named_property(env, o, "min", mn);
+  napi_set_named_property(env, o, "max", mx);
+  return o;
+}
+
+napi_value args(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
+  size_t argc = want;
+  napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+  if (argc < want) {
+    napi_throw_type_error(env, nullptr, "not enough arguments");
+    return nullptr;
+  }
+  return argv[0];
+}
+
+// create(dims {nx, ny, nz}, params) -> handle
+napi_value Create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  double nx = 0, ny = 0, nz = 0;
+  get_prop_num(env, argv[0], "nx", &nx);
+  get_prop_num(env, argv[0], "ny", &ny);
+  get_prop_num(env, argv[0], "nz", &nz);
+  wt_dims d{(int32_t)nx, (int32_t)ny, (int32_t)nz};
+  wt_params p{};
+  p.tau_wall = 0.5;
+  p.interp = 1;
+  parse_params(env, argv[1], &p);
+  wt_solver* s = nullptr;
+  int rc = wt_create(&d, &p, &s);
+  if (rc) return throw_wt(env, rc);
+  auto* h = new Handle{s};
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, h, finalize_handle, nullptr, &ext));
+  return ext;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  void* p = nullptr;
+  napi_get_value_external(env, argv[0], &p);
+  auto* h = static_cast<Handle*>(p);
+  if (h && h->s) {
+    wt_destroy(h->s);
+    h->s = nullptr;
+  }
+  return nullptr;
+}
+
+napi_value SetParams(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  wt_params p{};
+  wt_get_params(h->s, &p);
+  const uint32_t mask = parse_params(env, argv[1], &p);
+  int rc = wt_set_params(h->s, &p, mask);
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+
+napi_value Simple(napi_env env, napi_callback_info info, int (*fn)(wt_solver*)) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  int rc = fn(h->s);
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+napi_value Reset(napi_env env, napi_callback_info info) { return Simple(env, info, wt_reset); }
+napi_value ClearStats(napi_env env, napi_callback_info info) { return Simple(env, info, wt_clear_stats); }
+napi_value Sync(napi_env env, napi_callback_info info) { return Simple(env, info, wt_sync); }
+
+napi_value Info(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  wt_info i{};
+  int rc = wt_info_get(h->s, &i);
+  if (rc) return throw_wt(env, rc);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "nx", i.dims.nx);
+  set_num(env, o, "ny", i.dims.ny);
+  set_num(env, o, "nz", i.dims.nz);
+  set_num(env, o, "n", (double)i.n);
+  set_num(env, o, "q", i.q);
+  set_num(env, o, "precision", i.precision);
+  set_num(env, o, "step", (double)i.step);
+  set_num(env, o, "wallCells", (double)i.wall_cells);
+  set_num(env, o, "deviceBytes", (double)i.device_bytes);
+  set_num(env, o, "tau0", wt_tau0(h->s));
+  return o;
+}
+
+napi_value UploadFlags(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  uint32_t* d;
+  size_t len;
+  if (!typed_array(env, argv[1], napi_uint32_array, &d, &len)) {
+    napi_throw_type_error(env, nullptr, "flags must be a Uint32Array");
+    return nullptr;
+  }
+  wt_info i{};
+  wt_info_get(h->s, &i);
+  if ((int64_t)len != i.n) {
+    napi_throw_range_error(env, nullptr, "flags length must equal the cell count");
+    return nullptr;
+  }
+  int rc = wt_upload_flags(h->s, d);
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+
+napi_value ReadFlags(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  wt_info i{};
+  wt_info_get(h->s, &i);
+  void* data;
+  napi_value out = make_typed(env, napi_uint32_array, (size_t)i.n, 4, &data);
+  int rc = wt_read_flags(h->s, (uint32_t*)data);
+  if (rc) return throw_wt(env, rc);
+  return out;
+}
+
+napi_value UploadSdf(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  float* d;
+  size_t len;
+  if (!typed_array(env, argv[1], napi_float32_array, &d, &len)) {
+    napi_throw_type_error(env, nullptr, "sdf must be a Float32Array");
+    return nullptr;
+  }
+  wt_info i{};
+  wt_info_get(h->s, &i);
+  if ((int64_t)len != i.n) {
+    napi_throw_range_error(env, nullptr, "sdf length must equal the cell count");
+    return nullptr;
+  }
+  int rc = wt_upload_sdf(h->s, d);
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+
+// setWheels(h, Float32Array of 8 per wheel)
+napi_value SetWheels(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  float* d;
+  size_t len;
+  if (!typed_array(env, argv[1], napi_float32_array, &d, &len) || len % 8 != 0) {
+    napi_throw_type_error(env, nullptr, "wheels must be a Float32Array of 8 floats per wheel");
+    return nullptr;
+  }
+  int rc = wt_set_wheels(h->s, len ? d : nullptr, (int32_t)(len / 8));
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+
+napi_value Step(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  int32_t n = 0;
+  napi_get_value_int32(env, argv[1], &n);
+  int rc = wt_step(h->s, n);
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+
+struct SampleWork {
+  napi_async_work work;
+  napi_deferred deferred;
+  wt_solver* s;
+  int32_t steps;
+  int rc;
+  std::string err;
+  wt_stats st;
+};
+
+void sample_execute(napi_env, void* data) {
+  auto* w = static_cast<SampleWork*>(data);
+  w->rc = wt_step_and_sample(w->s, w->steps, &w->st);
+  if (w->rc) w->err = wt_last_error();
+}
+
+void sample_complete(napi_env env, napi_status, void* data) {
+  auto* w = static_cast<SampleWork*>(data);
+  if (w->rc) {
+    napi_value msg, err;
+    std::string m = "libwt error " + std::to_string(w->rc) + ": " + w->err;
+    napi_create_string_utf8(env, m.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_reject_deferred(env, w->deferred, err);
+  } else {
+    napi_resolve_deferred(env, w->deferred, stats_object(env, w->st));
+  }
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+// stepAndSample(h, steps) -> Promise<SolverStats>
+napi_value StepAndSample(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  auto* w = new SampleWork();
+  w->s = h->s;
+  napi_get_value_int32(env, argv[1], &w->steps);
+  napi_value promise, name;
+  NAPI_CALL(env, napi_create_promise(env, &w->deferred, &promise));
+  napi_create_string_utf8(env, "wt_step_and_sample", NAPI_AUTO_LENGTH, &name);
+  NAPI_CALL(env, napi_create_async_work(env, nullptr, name, sample_execute, sample_complete, w, &w->work));
+  NAPI_CALL(env, napi_queue_async_work(env, w->work));
+  return promise;
+}
+
+// stepAndSampleSync(h, steps) -> SolverStats (blocking variant)
+napi_value StepAndSampleSync(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  int32_t n = 0;
+  napi_get_value_int32(env, argv[1], &n);
+  wt_stats st{};
+  int rc = wt_step_and_sample(h->s, n, &st);
+  if (rc) return throw_wt(env, rc);
+  return stats_object(env, st);
+}
+
+napi_value ReadPopulations(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  wt_info i{};
+  wt_info_get(h->s, &i);
+  void* data;
+  napi_value out = make_typed(env, napi_float32_array, (size_t)(i.q * i.n), 4, &data);
+  int rc = wt_read_populations(h->s, (float*)data);
+  if (rc) return throw_wt(env, rc);
+  return out;
+}
+
+napi_value UploadPopulations(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  float* d;
+  size_t len;
+  wt_info i{};
+  wt_info_get(h->s, &i);
+  if (!typed_array(env, argv[1], napi_float32_array, &d, &len) || (int64_t)len != i.q * i.n) {
+    napi_throw_type_error(env, nullptr, "populations must be a Float32Array of q * n values");
+    return nullptr;
+  }
+  int rc = wt_upload_populations(h->s, d);
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+
+// voxelizeSdf(h, Float32Array prims packed as 20 floats each: kind, flags, c[3], rot[9], half[3], radius, param, pad)
+napi_value VoxelizeSdf(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  float* d;
+  size_t len;
+  if (!typed_array(env, argv[1], napi_float32_array, &d, &len) || len % 20 != 0) {
+    napi_throw_type_error(env, nullptr, "prims must be a Float32Array of 20 floats per primitive");
+    return nullptr;
+  }
+  std::vector<wt_prim> prims(len / 20);
+  for (size_t k = 0; k < prims.size(); k++) {
+    const float* e = d + 20 * k;
+    wt_prim& p = prims[k];
+    p.kind = (int32_t)e[0];
+    p.flags = (int32_t)e[1];
+    for (int a = 0; a < 3; a++) p.center[a] = e[2 + a];
+    for (int a = 0; a < 9; a++) p.rot[a] = e[5 + a];
+    for (int a = 0; a < 3; a++) p.half[a] = e[14 + a];
+    p.radius = e[17];
+    p.param = e[18];
+  }
+  wt_voxel_info vi{};
+  int rc = wt_voxelize_sdf(h->s, prims.data(), (int32_t)prims.size(), &vi);
+  if (rc) return throw_wt(env, rc);
+  return voxel_object(env, vi);
+}
+
+// voxelizeMesh(h, positions Float32Array, indices Uint32Array, matrix Float32Array(16), wheels Float32Array)
+napi_value VoxelizeMesh(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!args(env, info, 5, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  float *pos, *m, *wh;
+  uint32_t* idx;
+  size_t np, ni, nm, nw;
+  if (!typed_array(env, argv[1], napi_float32_array, &pos, &np) || !typed_array(env, argv[2], napi_uint32_array, &idx, &ni) ||
+      !typed_array(env, argv[3], napi_float32_array, &m, &nm) || nm != 16 || !typed_array(env, argv[4], napi_float32_array, &wh, &nw) ||
+      np % 3 || ni % 3 || nw % 8) {
+    napi_throw_type_error(env, nullptr, "voxelizeMesh(h, Float32Array pos, Uint32Array idx, Float32Array(16), Float32Array wheels)");
+    return nullptr;
+  }
+  wt_voxel_info vi{};
+  int rc = wt_voxelize_mesh(h->s, pos, (int32_t)(np / 3), idx, (int32_t)(ni / 3), m, nw ? wh : nullptr, (int32_t)(nw / 8), &vi);
+  if (rc) return throw_wt(env, rc);
+  return voxel_object(env, vi);
+}
+
+napi_value BytesFor(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  double nx = 0, ny = 0, nz = 0;
+  get_prop_num(env, argv[0], "nx", &nx);
+  get_prop_num(env, argv[0], "ny", &ny);
+  get_prop_num(env, argv[0], "nz", &nz);
+  wt_dims d{(int32_t)nx, (int32_t)ny, (int32_t)nz};
+  napi_value out;
+  napi_create_double(env, (double)wt_bytes_for(&d), &out);
+  return out;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  struct { const char* name; napi_callback cb; } fns[] = {
+      {"create", Create}, {"destroy", Destroy}, {"setParams", SetParams}, {"reset", Reset},
+      {"clearStats", ClearStats}, {"sync", Sync}, {"info", Info}, {"uploadFlags", UploadFlags},
+      {"readFlags", ReadFlags}, {"uploadSdf", UploadSdf}, {"setWheels", SetWheels}, {"step", Step},
+      {"stepAndSample", StepAndSample}, {"stepAndSampleSync", StepAndSampleSync},
+      {"readPopulations", ReadPopulations}, {"uploadPopulations", UploadPopulations},
+      {"voxelizeSdf", VoxelizeSdf}, {"voxelizeMesh", VoxelizeMesh}, {"bytesFor", BytesFor},
+  };
+  for (auto& f : fns) {
+    napi_value fn;
+    NAPI_CALL(env, napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));
+    NAPI_CALL(env, napi_set_named_property(env, exports, f.name, fn));
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)
