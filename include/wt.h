@@ -149,6 +149,9 @@ int wt_upload_flags(wt_solver* s, const uint32_t* flags);
 int wt_read_flags(wt_solver* s, uint32_t* out);
 /* uploadSdf(Float32Array), SolverGPU.ts:312-322: wall distance in cells, < 0 (or NaN) = unknown. */
 int wt_upload_sdf(wt_solver* s, const float* dist);
+/* No reference counterpart (a debug readback beside readFlags): the stored wall distances, decoded,
+ * -1 where unknown. */
+int wt_read_sdf(wt_solver* s, float* dist);
 /* setWheels(list), SolverGPU.ts:325-335: 8 floats per wheel (cx, cy, cz, ax, ay, az, r, hw),
  * grid space, at most 16 (extra wheels are ignored, as in the reference). */
 int wt_set_wheels(wt_solver* s, const float* wheels, int32_t count);

@@ -207,3 +207,37 @@ def round_half(x):
 
 def round_bf16(x):
     return load().or_round_bf16(float(x))
+
+
+def voxelize_mesh(positions, indices, matrix, dims, wheels=()):
+    """CPU restatement of the mesh voxelizer. Returns (flags u32[n], sdf_bits u32[n], info dict)."""
+    lib = load()
+    fn = lib.or_voxelize_mesh
+    fn.restype = None
+    P = C.c_void_p
+    fn.argtypes = [P, C.c_int, P, C.c_int, P, P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P]
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+    m = np.ascontiguousarray(matrix, dtype=np.float32).reshape(16)
+    wl = []
+    for w in list(wheels)[:16]:
+        wl += [w["c"][0], w["c"][1], w["c"][2], w["axis"][0], w["axis"][1], w["axis"][2], w["r"], w["hw"]]
+    wa = np.array(wl if wl else [0.0], dtype=np.float32)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    flags = np.zeros(n, np.uint32)
+    sdf = np.zeros(n, np.uint32)
+    info = np.zeros(8, np.int64)
+    fn(_ptr(pos), pos.size // 3, _ptr(idx), idx.size // 3, _ptr(m), _ptr(wa), len(wl) // 8, nx, ny, nz, _ptr(flags), _ptr(sdf),
+       _ptr(info))
+    ok = info[1] > 0
+    return flags, sdf, dict(frontal=int(info[0]), solidCells=int(info[1]),
+                            min=[int(v) for v in info[2:5]] if ok else [0, 0, 0],
+                            max=[int(v) + 1 for v in info[5:8]] if ok else [0, 0, 0])
+
+
+def decode_sdf(bits):
+    """~bits encoding -> distances (-1 unknown), SolverGPU.ts:312-322 / wgslLbm.ts:32-36."""
+    b = np.asarray(bits, dtype=np.uint32)
+    d = (~b).view(np.float32).astype(np.float64)
+    return np.where(b == 0, -1.0, d)

@@ -81,8 +81,11 @@ def test_stats_parity(name, prec):
     vo, mao, mbo = o.readMacro()
     # velocity texture is half precision (rgba16float in the reference)
     assert np.allclose(vg, vo, rtol=2e-3, atol=2e-4)
-    assert np.allclose(mag, mao, rtol=1e-3, atol=1e-5)
-    assert np.allclose(mbg, mbo, rtol=1e-3, atol=1e-7)
+    # EMA fields: FP32 state; with FP16 storage the velocity inherits the population error
+    # (~5e-4 of f over |u| ~ 0.03) and u^2 doubles it
+    r = 1e-4 if prec == "f32" else 5e-3
+    assert np.allclose(mag, mao, rtol=r, atol=r * np.abs(mao).max())
+    assert np.allclose(mbg, mbo, rtol=2 * r, atol=2 * r * np.abs(mbo).max())
 
 
 @pytest.mark.parametrize("prec", ["f32", "f16", "bf16"])

@@ -85,6 +85,7 @@ SIGNATURES = {
     "wt_upload_flags": (C.c_int, [_P, _P]),
     "wt_read_flags": (C.c_int, [_P, _P]),
     "wt_upload_sdf": (C.c_int, [_P, _P]),
+    "wt_read_sdf": (C.c_int, [_P, _P]),
     "wt_set_wheels": (C.c_int, [_P, _P, C.c_int32]),
     "wt_voxelize_sdf": (C.c_int, [_P, C.POINTER(Prim), C.c_int32, C.POINTER(VoxelInfo)]),
     "wt_voxelize_mesh": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P, _P, C.c_int32, C.POINTER(VoxelInfo)]),

@@ -25,6 +25,10 @@ HIP_FLAGS = [
 ]
 
 
+# the voxelizer is compiled without contraction so the CPU restatement reproduces it exactly
+EXTRA_FLAGS = {"kernels_vox.hip": ["-ffp-contract=off"]}
+
+
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
@@ -40,7 +44,8 @@ def build_lib(verbose=False):
     for src in SOURCES:
         obj = os.path.join(LIB_DIR, "obj", src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        extra = EXTRA_FLAGS.get(src, [])
+        cmds.append([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():

@@ -82,6 +82,35 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const uint32_t* __restrict__
 // Groups of CPT cells along x (the bulk kernel's unit) that contain a WALL cell, or mix solid and
 // fluid, go to the wall kernel; their cells are listed with their solid masks.
 // counts[0] = listed cells, counts[1] = bulk fluid cells, counts[2] = bulk solid cells.
+// Per-link data of a wall cell, fixed by the geometry: the Bouzidi fraction q of every solid link
+// (with the reference's fallback to q = 1/2 when the second fluid node is missing folded in,
+// wgslLbm.ts:172) and the wheel tag of the solid cell it points into.
+template <class L>
+__device__ WallRec wall_record(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sdf, int64_t c, int ix, int iy,
+                               int iz, uint32_t m, KParams P) {
+  P.interp = 1;  // the record holds the interpolated q; halfway walls select 1/2 at step time
+  WallRec R;
+#pragma unroll
+  for (int i = 0; i < 20; i++) { R.q[i < 19 ? i : 18] = 0.5f; R.tag[i] = 0; }
+#pragma unroll
+  for (int i = 1; i < L::Q; i++) {
+    if (!(m & (1u << i))) continue;
+    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
+    const int sx = cx < 0 ? min(ix - cx, P.nx - 1) : ix - cx;
+    const int64_t sc = (int64_t)sx + (int64_t)P.nx * ((int64_t)(iy - cy) + (int64_t)P.ny * (L::DIM == 3 ? iz - cz : 0));
+    float q = link_q(sdf, P, c, sc);
+    if (q < 0.5f) {
+      const int ax = ix + cx, ay = iy + cy, az = L::DIM == 3 ? iz + cz : 0;
+      const bool ok = ax >= 0 && ay >= 0 && az >= 0 && ax < P.nx && ay < P.ny && az < P.nz &&
+                      flags[(int64_t)ax + (int64_t)P.nx * ((int64_t)ay + (int64_t)P.ny * az)] == 0u;
+      if (!ok) q = 0.5f;
+    }
+    R.q[i] = q;
+    R.tag[i] = (uint8_t)(flags[sc] >> 8);
+  }
+  return R;
+}
+
 template <class L, int CPT>
 __global__ __launch_bounds__(BLOCK) void k_collect(const uint8_t* __restrict__ kind, const uint32_t* __restrict__ flags,
                                                    uint32_t* __restrict__ list, uint32_t* __restrict__ masks,
@@ -136,6 +165,19 @@ __global__ __launch_bounds__(BLOCK) void k_collect(const uint8_t* __restrict__ k
   }
 }
 
+// Per-link records of the listed wall cells; rerun whenever flags or distances change.
+template <class L>
+__global__ __launch_bounds__(BLOCK) void k_wallprep(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sdf,
+                                                    const uint32_t* __restrict__ list, const uint32_t* __restrict__ masks,
+                                                    WallRec* __restrict__ recs, int64_t count, KParams P) {
+  const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= count) return;
+  const int64_t c = list[t];
+  int ix, iy, iz;
+  coords(c, P, ix, iy, iz);
+  recs[t] = wall_record<L>(flags, sdf, c, ix, iy, iz, masks[t], P);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Bulk kernel. Block = 4 waves; each wave owns a chunk of 64 * CPT cells of one x-row, so the row
 // (y, z) — and with it every face condition (ground, mirrors, periodic z, y-z edges) — is uniform
@@ -163,11 +205,54 @@ __global__ __launch_bounds__(BLOCK) void k_bulk(const typename S::T* __restrict_
   const int iy = (int)row - iz * NY;
   const int64_t rowoff = (int64_t)row * NX;
   const int64_t c0 = rowoff + x0;
+  // the kind byte(s) are loaded together with the populations: every source address below is in
+  // bounds whatever the kind, so the loads need not wait for the skip test
   uint32_t k;
   if constexpr (CPT == 2) k = *(const uint16_t*)(kind + c0);
   else k = kind[c0];
   constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
+  const bool yLo = iy == 0, yHi = iy == NY - 1;
+  const bool zLo = D3 && iz == 0, zHi = D3 && iz == NZ - 1;
+  const bool gwall = yLo && P.ground != 0;
+  // ---- 1. per direction: source direction, row and x shift (wave-uniform, branch-free) ----
+  const T* src[Q];
+  int xlim[Q];  // largest x offset a lane may use (outlet clamp for CPT = 1)
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
+    const bool yOut = (cy > 0 && yLo) || (cy < 0 && yHi);
+    const bool zOut = (cz > 0 && zLo) || (cz < 0 && zHi);
+    const bool grnd = cy > 0 && yOut && P.ground != 0;
+    const bool yz = D3 && yOut && zOut;
+    const bool zper = P.sideMode == 1;
+    // source direction: opp (ground), mirrorYZ, mirrorY, i / mirrorZ (z faces), i
+    const int d = grnd ? opp<L>(i) : yz ? mirYZ<L>(i) : yOut ? mirY<L>(i) : (zOut && !zper) ? mirZ<L>(i) : i;
+    const int sy = (grnd || yOut) ? iy : iy - cy;
+    const int szp = D3 ? ((iz - cz + NZ) % NZ) : 0;
+    const int sz = (grnd || yz) ? iz : yOut ? (D3 ? min(max(iz - cz, 0), NZ - 1) : 0) : zOut ? (zper ? szp : iz) : iz - cz;
+    const int xs = grnd ? 0 : -cx;
+    src[i] = fin + (int64_t)d * n + ((int64_t)sy + (int64_t)NY * sz) * NX + xs;
+    xlim[i] = (grnd || cx >= 0) ? NX - 1 : NX - 2;
+  }
+  // ---- 2. all loads in flight together ----
+  float fa[Q], fb[Q];
+  if constexpr (CPT == 2) {
+    typename S::Word raw[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) raw[i] = S::ld_word(src[i] + x0);
+#pragma unroll
+    for (int i = 0; i < Q; i++) S::unpack_word(raw[i], L::w(i), fa[i], fb[i]);
+  } else {
+    T raw[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) raw[i] = src[i][min(x0, xlim[i])];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      fa[i] = S::dev_of(raw[i], L::w(i));
+      fb[i] = 0.0f;
+    }
+  }
   if (k == SOLID_ALL) {
 #pragma unroll
     for (int i = 0; i < Q; i++) {
@@ -178,47 +263,23 @@ __global__ __launch_bounds__(BLOCK) void k_bulk(const typename S::T* __restrict_
     return;
   }
   if (k & SKIP) return;
-  const bool yLo = iy == 0, yHi = iy == NY - 1;
-  const bool zLo = D3 && iz == 0, zHi = D3 && iz == NZ - 1;
-  const bool gwall = yLo && P.ground != 0;
+  // ---- 3. per-lane fix-ups: outlet clamp, moving belt, inlet ----
   const float beltA = (P.ground == 2 && !(k & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
   const float beltB = (P.ground == 2 && !((k >> 8) & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
   const float U = P.uin;
-  float fa[Q], fb[Q];
 #pragma unroll
-  for (int i = 0; i < Q; i++) {
-    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-    // ---- source direction and row (wave-uniform) ----
-    const bool yOut = (cy > 0 && yLo) || (cy < 0 && yHi);
-    const bool zOut = (cz > 0 && zLo) || (cz < 0 && zHi);
-    const bool grnd = cy > 0 && yOut && P.ground != 0;
-    int d, sy, sz, xs;
-    if (i == 0) { d = 0; sy = iy; sz = iz; xs = 0; }
-    else if (grnd) { d = opp<L>(i); sy = iy; sz = iz; xs = 0; }
-    else if (D3 && yOut && zOut) { d = mirYZ<L>(i); sy = iy; sz = iz; xs = -cx; }
-    else if (yOut) { d = mirY<L>(i); sy = iy; sz = D3 ? min(max(iz - cz, 0), NZ - 1) : 0; xs = -cx; }
-    else if (D3 && zOut) {
-      if (P.sideMode == 1) { d = i; sy = iy - cy; sz = (iz - cz + NZ) % NZ; }
-      else { d = mirZ<L>(i); sy = iy - cy; sz = iz; }
-      xs = -cx;
-    } else { d = i; sy = iy - cy; sz = iz - cz; xs = -cx; }
-    const T* base = fin + (int64_t)d * n + ((int64_t)sy + (int64_t)NY * sz) * NX;
-    float va, vb = 0.0f;
+  for (int i = 1; i < Q; i++) {
+    const int cx = L::c(i, 0), cy = L::c(i, 1);
+    const bool grnd = cy > 0 && yLo && P.ground != 0;
     if constexpr (CPT == 2) {
-      S::template ld2_dev<false>(base + (x0 + xs), L::w(i), va, vb);
-      if (cx < 0 && !grnd && x0 + 2 == NX) vb = va;  // outlet: x = nx clamps to nx - 1
-    } else {
-      const int xl = min(x0 + xs, NX - 1);
-      va = S::ld_dev(base, xl, L::w(i));
+      if (cx < 0 && !grnd && x0 + 2 == NX) fb[i] = fa[i];  // outlet: x = nx clamps to nx - 1
     }
     if (grnd && cx != 0) {  // moving belt, wgslLbm.ts:159-162
       const float m = (6.0f * L::w(i) * cx) * P.uGround;
-      va += beltA * m;
-      vb += beltB * m;
+      fa[i] += beltA * m;
+      fb[i] += beltB * m;
     }
-    if (cx > 0 && x0 == 0) va = inlet_dev<L>(i, U);  // inlet, wgslLbm.ts:146 (checked first there)
-    fa[i] = va;
-    fb[i] = vb;
+    if (cx > 0 && x0 == 0) fa[i] = inlet_dev<L>(i, U);  // inlet, wgslLbm.ts:146 (checked first there)
   }
   collide<L>(fa, (float)x0, gwall, gwall, P);
   if constexpr (CPT == 2) collide<L>(fb, (float)(x0 + 1), gwall, gwall, P);
@@ -234,16 +295,16 @@ __global__ __launch_bounds__(BLOCK) void k_bulk(const typename S::T* __restrict_
 // logic with precomputed solid masks.
 template <class L, class S>
 __global__ __launch_bounds__(BLOCK) void k_wall(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                                const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sdf,
                                                 const uint8_t* __restrict__ kind, const uint32_t* __restrict__ list,
-                                                const uint32_t* __restrict__ masks, int64_t count, KParams P, KWheels W) {
+                                                const uint32_t* __restrict__ masks, const WallRec* __restrict__ recs,
+                                                int64_t count, KParams P, KWheels W) {
   const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (t >= count) return;
   const int64_t c = list[t];
   int ix, iy, iz;
   coords(c, P, ix, iy, iz);
   float f[L::Q];
-  update_wall<L, S>(fin, flags, sdf, kind[c], masks[t], P, W, c, ix, iy, iz, f);
+  update_wall<L, S>(fin, kind[c], masks[t], recs[t], P, W, c, ix, iy, iz, f);
 #pragma unroll
   for (int i = 0; i < L::Q; i++) S::st_dev(fout, (int64_t)i * P.n + c, L::w(i), f[i]);
 }
@@ -477,12 +538,22 @@ void launch_classify(const LaunchCtx& c, uint32_t* list, uint32_t* masks, unsign
     using L = decltype(l);
     hipLaunchKernelGGL((k_classify<L>), dim3(blocks_for(c.P.n)), dim3(BLOCK), 0, c.stream, c.flags, c.kind, c.P);
     if (cpt == 2)
-      hipLaunchKernelGGL((k_collect<L, 2>), dim3(blocks_for(ngroups)), dim3(BLOCK), 0, c.stream, c.kind, c.flags, list, masks, counts, c.P);
+      hipLaunchKernelGGL((k_collect<L, 2>), dim3(blocks_for(ngroups)), dim3(BLOCK), 0, c.stream, c.kind, c.flags, list, masks,
+                         counts, c.P);
     else
-      hipLaunchKernelGGL((k_collect<L, 1>), dim3(blocks_for(ngroups)), dim3(BLOCK), 0, c.stream, c.kind, c.flags, list, masks, counts, c.P);
+      hipLaunchKernelGGL((k_collect<L, 1>), dim3(blocks_for(ngroups)), dim3(BLOCK), 0, c.stream, c.kind, c.flags, list, masks,
+                         counts, c.P);
   };
   if (c.P.q == 9) go(D2Q9{});
   else go(D3Q19{});
+}
+
+void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, WallRec* recs, int64_t count) {
+  if (count <= 0) return;
+  if (c.P.q == 9)
+    hipLaunchKernelGGL((k_wallprep<D2Q9>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, c.flags, c.sdf, list, masks, recs, count, c.P);
+  else
+    hipLaunchKernelGGL((k_wallprep<D3Q19>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, c.flags, c.sdf, list, masks, recs, count, c.P);
 }
 
 void launch_bulk(const LaunchCtx& c, const void* fin, void* fout) {
@@ -499,12 +570,13 @@ void launch_bulk(const LaunchCtx& c, const void* fin, void* fout) {
   });
 }
 
-void launch_wall(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, const uint32_t* masks, int64_t count) {
+void launch_wall(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, const uint32_t* masks, const WallRec* recs,
+                 int64_t count) {
   if (count <= 0) return;
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
     hipLaunchKernelGGL((k_wall<L, S>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, (const typename S::T*)fin,
-                       (typename S::T*)fout, c.flags, c.sdf, c.kind, list, masks, count, c.P, *c.W);
+                       (typename S::T*)fout, c.kind, list, masks, recs, count, c.P, *c.W);
   });
 }
 

@@ -16,8 +16,10 @@ struct LaunchCtx {
 
 void launch_init(const LaunchCtx& c, void* f);
 void launch_classify(const LaunchCtx& c, uint32_t* list, uint32_t* masks, unsigned long long* counts);
+void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, WallRec* recs, int64_t count);
 void launch_bulk(const LaunchCtx& c, const void* fin, void* fout);
-void launch_wall(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, const uint32_t* masks, int64_t count);
+void launch_wall(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, const uint32_t* masks, const WallRec* recs,
+                 int64_t count);
 int cells_per_thread(const KParams& P);
 int64_t macro_blocks(int64_t n);
 void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, float* meanB, double* partials, double* result);

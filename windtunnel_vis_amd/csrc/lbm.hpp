@@ -105,6 +105,14 @@ struct SF32 {
     f32x2a4 v = {w + a, w + b};
     __builtin_nontemporal_store(v, (f32x2a4*)p);
   }
+  // raw access for the bulk kernel: a word holds two consecutive cells
+  using Word = uint64_t;
+  __device__ static Word ld_word(const T* p) { return *(const u64_a4*)p; }
+  __device__ static void unpack_word(Word v, float w, float& a, float& b) {
+    a = __uint_as_float((uint32_t)v) - w;
+    b = __uint_as_float((uint32_t)(v >> 32)) - w;
+  }
+  __device__ static float dev_of(T v, float w) { return v - w; }
 };
 
 struct SF16 {
@@ -121,6 +129,10 @@ struct SF16 {
     unpack_h2(v, a, b);
   }
   __device__ static void st2_dev(T* p, float, float a, float b) { __builtin_nontemporal_store(pack_h2(a, b), (uint32_t*)p); }
+  using Word = uint32_t;
+  __device__ static Word ld_word(const T* p) { return *(const u32_a2*)p; }
+  __device__ static void unpack_word(Word v, float, float& a, float& b) { unpack_h2(v, a, b); }
+  __device__ static float dev_of(T v, float) { return h2f(v); }
 };
 
 struct SBF16 {
@@ -137,6 +149,10 @@ struct SBF16 {
     unpack_b2(v, a, b);
   }
   __device__ static void st2_dev(T* p, float, float a, float b) { __builtin_nontemporal_store(pack_b2(a, b), (uint32_t*)p); }
+  using Word = uint32_t;
+  __device__ static Word ld_word(const T* p) { return *(const u32_a2*)p; }
+  __device__ static void unpack_word(Word v, float, float& a, float& b) { unpack_b2(v, a, b); }
+  __device__ static float dev_of(T v, float) { return b2f(v); }
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -177,6 +193,12 @@ struct KParams {
 struct KWheels {
   int32_t count, pad0, pad1, pad2;
   float4 w[32];  // w[2k] = (centre, 1/R), w[2k+1] = (axis, half width), SolverGPU.ts:325-335
+};
+
+// Precomputed per-link data of a cell next to the body (see wall_record in kernels_lbm.hip).
+struct WallRec {
+  float q[19];       // Bouzidi fraction per direction (1/2 for halfway links and non-solid links)
+  uint8_t tag[20];   // wheel index + 1 of the solid cell a link points into (0: not a wheel)
 };
 
 // wgslLbm.ts:21-28
@@ -325,9 +347,8 @@ __device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall,
 // solid cell (precomputed from flags). Produces post-collision deviations.
 // ---------------------------------------------------------------------------------------------
 template <class L, class S>
-__device__ inline void update_wall(const typename S::T* __restrict__ fin, const uint32_t* __restrict__ flags,
-                                   const uint32_t* __restrict__ sdf, uint8_t kd, uint32_t smask, const KParams& P,
-                                   const KWheels& W, int64_t idx, int ix, int iy, int iz, float (&f)[L::Q]) {
+__device__ inline void update_wall(const typename S::T* __restrict__ fin, uint8_t kd, uint32_t smask, const WallRec& R,
+                                   const KParams& P, const KWheels& W, int64_t idx, int ix, int iy, int iz, float (&f)[L::Q]) {
   constexpr int Q = L::Q, DIM = L::DIM;
   const int64_t n = P.n;
   const int NX = P.nx, NY = P.ny, NZ = P.nz;
@@ -371,23 +392,21 @@ __device__ inline void update_wall(const typename S::T* __restrict__ fin, const 
         const int64_t sc = cell(sxc, sy, sz);
         if (smask & (1u << i)) {
           wall = true;
+          // Bouzidi (q < 1/2 with the second fluid node x + c_i, q > 1/2 with f_i(x)); the record
+          // already holds q = 1/2 where the reference falls back to halfway bounce-back
           const float fb = S::ld_dev(fin, (int64_t)opp<L>(i) * n + idx, w);
-          float q = link_q(sdf, P, idx, sc);
+          const float q = P.interp ? R.q[i] : 0.5f;
           v = fb;
           if (q < 0.5f) {
             const int ax = ix + cx, ay = iy + cy, az = DIM == 3 ? iz + cz : 0;
-            if (ax >= 0 && ay >= 0 && az >= 0 && ax < NX && ay < NY && az < NZ && flags[cell(ax, ay, az)] == 0u)
-              v = 2.0f * q * fb + (1.0f - 2.0f * q) * S::ld_dev(fin, (int64_t)opp<L>(i) * n + cell(ax, ay, az), w);
-            else
-              q = 0.5f;
+            v = 2.0f * q * fb + (1.0f - 2.0f * q) * S::ld_dev(fin, (int64_t)opp<L>(i) * n + cell(ax, ay, az), w);
           } else if (q > 0.5f) {
             v = (0.5f / q) * fb + (1.0f - 0.5f / q) * S::ld_dev(fin, (int64_t)i * n + idx, w);
           }
-          const uint32_t fs = flags[sc];
-          if ((fs >> 8) != 0u) {
+          if (R.tag[i] != 0) {
             const float3 p = make_float3(0.5f * (float)(ix + sxc) + 0.5f, 0.5f * (float)(iy + sy) + 0.5f,
                                          0.5f * (float)(iz + sz) + 0.5f);
-            const float3 uw = wheel_vel(W, P, (fs >> 8) - 1u, p);
+            const float3 uw = wheel_vel(W, P, (uint32_t)R.tag[i] - 1u, p);
             const float mv = (6.0f * w) * ((float)cx * uw.x + (float)cy * uw.y + (float)cz * uw.z) * (q > 0.5f ? 0.5f / q : 1.0f);
             v += mv; mvSum += mv; mvW += w; wm |= 1u << i;
           }

@@ -48,6 +48,8 @@ struct wt_solver {
   uint8_t* kind = nullptr;
   uint32_t* wlist = nullptr;   // cells handled by the wall kernel
   uint32_t* wmask = nullptr;   // their solid masks
+  WallRec* wrec = nullptr;     // their per-link records
+  int64_t wrec_cap = 0;
   unsigned long long* counts = nullptr;
   ushort4* vel = nullptr;
   float4* meanA = nullptr;
@@ -134,6 +136,14 @@ int run_init(wt_solver* s) {
   return WT_OK;
 }
 
+// Per-link records of the wall cells (depend on flags and distances).
+int prep_walls(wt_solver* s) {
+  LaunchCtx c = s->ctx();
+  launch_wallprep(c, s->wlist, s->wmask, s->wrec, s->nwall);
+  HIP_TRY(hipGetLastError());
+  return WT_OK;
+}
+
 int reclassify(wt_solver* s) {
   HIP_TRY(hipMemsetAsync(s->counts, 0, 4 * sizeof(unsigned long long), s->stream));
   LaunchCtx c = s->ctx();
@@ -145,7 +155,15 @@ int reclassify(wt_solver* s) {
   s->nwall = (int64_t)h[0];
   s->nfast = (int64_t)h[1];
   s->nsolid = (int64_t)h[2];
-  return WT_OK;
+  if (s->nwall > s->wrec_cap) {
+    if (s->wrec) HIP_TRY(hipFree(s->wrec));
+    s->wrec = nullptr;
+    s->wrec_cap = 0;
+    const int64_t cap = s->nwall + s->nwall / 4 + 1024;
+    if (hipMalloc((void**)&s->wrec, sizeof(WallRec) * (size_t)cap) != hipSuccess) return fail(WT_E_NOMEM, "wall record allocation failed");
+    s->wrec_cap = cap;
+  }
+  return prep_walls(s);
 }
 
 int do_steps(wt_solver* s, int steps) {
@@ -168,12 +186,12 @@ int do_steps(wt_solver* s, int steps) {
       HIP_TRY(hipEventRecord(e[0], s->stream));
       launch_bulk(c, fin, fout);
       HIP_TRY(hipEventRecord(e[1], s->stream));
-      launch_wall(c, fin, fout, s->wlist, s->wmask, s->nwall);
+      launch_wall(c, fin, fout, s->wlist, s->wmask, s->wrec, s->nwall);
       HIP_TRY(hipEventRecord(e[3], s->stream));
       e[2] = e[1];
     } else {
       launch_bulk(c, fin, fout);
-      launch_wall(c, fin, fout, s->wlist, s->wmask, s->nwall);
+      launch_wall(c, fin, fout, s->wlist, s->wmask, s->wrec, s->nwall);
     }
     s->parity ^= 1;
   }
@@ -268,7 +286,7 @@ int wt_destroy(wt_solver* s) {
   if (!s) return WT_OK;
   if (s->stream) hipStreamSynchronize(s->stream);
   for (auto e : s->ev) hipEventDestroy(e);
-  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
+  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wrec, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (s->h_result) hipHostFree(s->h_result);
@@ -358,7 +376,21 @@ int wt_upload_sdf(wt_solver* s, const float* d) {
     enc[i] = ~b;
   }
   HIP_TRY(hipMemcpyAsync(s->sdf, enc.data(), 4 * s->n, hipMemcpyHostToDevice, s->stream));
+  if (int r = prep_walls(s)) return r;
   HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+int wt_read_sdf(wt_solver* s, float* d) {
+  if (!s || !d) return fail(WT_E_ARG, "null argument");
+  std::vector<uint32_t> enc((size_t)s->n);
+  HIP_TRY(hipMemcpyAsync(enc.data(), s->sdf, 4 * s->n, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int64_t i = 0; i < s->n; i++) {
+    if (enc[i] == 0u) { d[i] = -1.0f; continue; }
+    const uint32_t b = ~enc[i];
+    std::memcpy(&d[i], &b, 4);
+  }
   return WT_OK;
 }
 

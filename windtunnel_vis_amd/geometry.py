@@ -211,3 +211,157 @@ def sphere_body(dims, center, radius):
     b.length = 2.0 * radius
     b.add(N.PRIM_SPHERE, center, None, (radius, 0, 0))
     return b
+
+
+# ---------------------------------------------------------------------------------------------
+# Triangle meshes for the mesh voxelizer (wt_voxelize_mesh).
+# ---------------------------------------------------------------------------------------------
+class Mesh:
+    """positions (V, 3) float32, indices (T, 3) uint32, optional wheels (mesh space), bbox."""
+
+    def __init__(self, positions, indices, wheels=None):
+        self.positions = np.asarray(positions, dtype=np.float32).reshape(-1, 3)
+        self.indices = np.asarray(indices, dtype=np.uint32).reshape(-1, 3)
+        self.wheels = wheels or []
+
+    @property
+    def min(self):
+        return self.positions.min(axis=0).astype(np.float64)
+
+    @property
+    def max(self):
+        return self.positions.max(axis=0).astype(np.float64)
+
+
+def uv_sphere(radius=0.5, width_segments=96, height_segments=64):
+    """Latitude-longitude sphere with outward-oriented triangles (the sphere preset is a 96 x 64
+    UV sphere of radius 0.5, presets.ts:366-368). Pole rows emit one triangle per quad."""
+    verts = []
+    for iy in range(height_segments + 1):
+        v = iy / height_segments
+        th = v * math.pi
+        for ix in range(width_segments + 1):
+            u = ix / width_segments
+            ph = u * 2 * math.pi
+            verts.append((-radius * math.cos(ph) * math.sin(th), radius * math.cos(th), radius * math.sin(ph) * math.sin(th)))
+    W = width_segments + 1
+    tris = []
+    for iy in range(height_segments):
+        for ix in range(width_segments):
+            a = iy * W + ix + 1
+            b = iy * W + ix
+            c = (iy + 1) * W + ix
+            d = (iy + 1) * W + ix + 1
+            if iy != 0:
+                tris.append((a, b, d))
+            if iy != height_segments - 1:
+                tris.append((b, c, d))
+    return Mesh(verts, tris)
+
+
+def box_mesh(lo, hi):
+    """Axis-aligned box with outward-oriented triangles."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    v = [(x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0), (x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)]
+    f = [(0, 2, 1), (0, 3, 2), (4, 5, 6), (4, 6, 7), (0, 1, 5), (0, 5, 4), (3, 7, 6), (3, 6, 2), (0, 4, 7), (0, 7, 3),
+         (1, 2, 6), (1, 6, 5)]
+    return Mesh(v, f)
+
+
+def cylinder_mesh(center, radius, half_width, segments=48):
+    """Closed cylinder with axis z (a wheel), outward-oriented."""
+    cx, cy, cz = center
+    v = []
+    for k in range(segments):
+        a = 2 * math.pi * k / segments
+        v.append((cx + radius * math.cos(a), cy + radius * math.sin(a), cz - half_width))
+        v.append((cx + radius * math.cos(a), cy + radius * math.sin(a), cz + half_width))
+    v.append((cx, cy, cz - half_width))
+    v.append((cx, cy, cz + half_width))
+    c0, c1 = 2 * segments, 2 * segments + 1
+    t = []
+    for k in range(segments):
+        a0, a1 = 2 * k, 2 * k + 1
+        b0, b1 = 2 * ((k + 1) % segments), 2 * ((k + 1) % segments) + 1
+        t += [(a0, b0, b1), (a0, b1, a1), (c0, b0, a0), (c1, a1, b1)]
+    return Mesh(v, t)
+
+
+def merge_meshes(meshes):
+    pos, idx, off = [], [], 0
+    wheels = []
+    for m in meshes:
+        pos.append(m.positions)
+        idx.append(m.indices + off)
+        off += len(m.positions)
+        wheels += m.wheels
+    return Mesh(np.concatenate(pos), np.concatenate(idx), wheels)
+
+
+def normalize_mesh(mesh, align_longest=True):
+    """mesh.ts:271-318 for a y-up mesh: longest horizontal extent along +x, length 1, centred in x/z,
+    lowest point at y = 0. Wheels follow the same transform (presets.ts:369-379)."""
+    p = mesh.positions.astype(np.float64)
+    T = np.eye(4)
+    lo, hi = p.min(0), p.max(0)
+    if align_longest and (hi[2] - lo[2]) > (hi[0] - lo[0]) * 1.05:
+        R = np.eye(4)
+        R[:3, :3] = rot_y(90)
+        p = p @ R[:3, :3].T
+        T = R @ T
+        lo, hi = p.min(0), p.max(0)
+    length = max(hi[0] - lo[0], 1e-9)
+    cx, cz = (hi[0] + lo[0]) / 2, (hi[2] + lo[2]) / 2
+    S = np.diag([1 / length, 1 / length, 1 / length, 1.0])
+    Tr = np.eye(4)
+    Tr[:3, 3] = [-cx, -lo[1], -cz]
+    M = S @ Tr
+    p = (p - np.array([cx, lo[1], cz])) / length
+    T = M @ T
+    wheels = []
+    for w in mesh.wheels:
+        c = (T @ np.array(list(w["c"]) + [1.0]))[:3]
+        a = T[:3, :3] @ np.asarray(w["axis"], dtype=np.float64)
+        a /= np.linalg.norm(a)
+        wheels.append(dict(c=list(c), axis=list(a), r=w["r"] / length, hw=w["hw"] / length))
+    return Mesh(p, mesh.indices, wheels)
+
+
+def placement_matrix(mesh, dims, mode="ground", length_frac=1 / 3, diam_frac=0.3, x_frac=0.34, yaw_deg=0.0,
+                     pitch_deg=0.0, ride_cells=0.0):
+    """Mesh -> grid matrix (column-major 16 floats) and scale (mesh.ts:354-379)."""
+    nx, ny, nz = dims
+    lo, hi = mesh.min, mesh.max
+    height = hi[1] - lo[1]
+    scale = length_frac * nx if mode == "ground" else diam_frac * ny / max(height, 1e-6)
+    cy = (hi[1] + lo[1]) / 2
+    R = np.eye(4)
+    R[:3, :3] = rot_y(yaw_deg) @ rot_z(-pitch_deg)
+    S = np.diag([scale, scale, scale, 1.0])
+    C = np.eye(4)
+    C[1, 3] = -cy
+    M = R @ S @ C
+    pts = np.c_[mesh.positions.astype(np.float64), np.ones(len(mesh.positions))] @ M.T
+    min_y = pts[:, 1].min()
+    zc = nz / 2 if nz > 1 else 0.5
+    ty = -min_y + ride_cells if mode == "ground" else ny / 2 + 0.37
+    T = np.eye(4)
+    T[:3, 3] = [x_frac * nx, ty, zc]
+    full = T @ M
+    return full.T.reshape(-1).astype(np.float32), scale
+
+
+def grid_wheels(mesh, matrix16):
+    """Wheels in grid space (mesh.ts:321-333)."""
+    M = np.asarray(matrix16, dtype=np.float64).reshape(4, 4).T
+    lin = M[:3, :3]
+    s = np.linalg.norm(lin[:, 0])
+    out = []
+    for w in mesh.wheels:
+        c = (M @ np.array(list(w["c"]) + [1.0]))[:3]
+        a = lin @ np.asarray(w["axis"], dtype=np.float64)
+        scale_axis = np.linalg.norm(a)
+        a /= scale_axis
+        out.append(dict(c=list(c), axis=list(a), r=w["r"] * s, hw=w["hw"] * scale_axis))
+    return out

@@ -158,6 +158,12 @@ class SolverGPU:
         assert a.size == self.n, "sdf must have n entries"
         N.check(self._lib.wt_upload_sdf(self._h, N.ptr(a)))
 
+    def readSdf(self):
+        """Stored wall distances (cells), -1 where unknown."""
+        out = np.empty(self.n, dtype=np.float32)
+        N.check(self._lib.wt_read_sdf(self._h, N.ptr(out)))
+        return out
+
     def setWheels(self, wheels):
         """wheels: list of dicts {c: [x,y,z], axis: [x,y,z], r, hw} in grid space (SolverGPU.ts:325)."""
         a = wheel_array(wheels)
