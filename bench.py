@@ -3,11 +3,11 @@
 Workload (default): the plain D3Q19 LES at 256x128x128 with FP16 deviation storage, Bouzidi walls,
 moving belt at u = 0.08 with rolling wheels, Cs = 0.1, around the procedural sedan (rounded box
 85x22x32 cells, corner radius 6, four wheels r = 6, width 5, ride height 3). One step = one fused
-pull-stream + collide pass over the whole grid (bulk kernel + boundary kernel).
+pull-stream + collide pass over the whole grid (one launch of the step kernel).
 
 Prints ONE JSON line. `value` is MLUPS over the timed steps with the populations resident in HBM.
-`roofline` prices the dominant kernel (the bulk stream-collide kernel) at 2*Q*s algorithmic bytes
-per cell it updates, divided by its average launch time measured with HIP events on the solver's
+`roofline` prices the step kernel (one launch per step: bulk and wall cells) at 2*Q*s algorithmic
+bytes per fluid cell, divided by its average launch time measured with HIP events on the solver's
 stream, against the 8 TB/s HBM peak. `cpu_baseline` times the CPU restatement on a bounded sample.
 """
 import argparse
@@ -53,8 +53,9 @@ def traffic_from_profiles(name):
     return None
 
 
-def run_gpu(dims, steps, warmup, precision="f16"):
+def run_gpu(dims, steps, warmup, precision="f16", xcd=1):
     s, body, info = make_sedan_solver(dims, precision)
+    s.setTuning(0, xcd)
     q = s.lattice.q
     sbytes = 2 if s.precision != "f32" else 4
     n = s.n
@@ -66,18 +67,27 @@ def run_gpu(dims, steps, warmup, precision="f16"):
     s.step(steps)
     s.sync()
     t1 = time.perf_counter()
-    bulk_ms, launches, wall_ms = s.profileRead()
+    k_ms, launches, _ = s.profileRead()
     s.profileEnable(False)
     st = s.stepAndSample(0)
     bulk_cells, wall_cells = s.workSplit()
     wall = t1 - t0
     mlups = n * steps / wall / 1e6
-    bulk_avg_s = bulk_ms / 1e3 / max(launches, 1)
-    bulk_bytes = bulk_cells * 2 * q * sbytes
-    achieved = bulk_bytes / bulk_avg_s / 1e9
-    return dict(solver=s, body=body, info=info, mlups=mlups, wall=wall, bulk_avg_ms=bulk_avg_s * 1e3,
-                wall_avg_ms=wall_ms / max(launches, 1), bulk_cells=bulk_cells, wall_cells=wall_cells, bulk_bytes=bulk_bytes,
-                achieved_gbs=achieved, unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes)
+    k_avg_s = k_ms / 1e3 / max(launches, 1)
+    fluid = n - info["solidCells"]
+    k_bytes = fluid * 2 * q * sbytes
+    achieved = k_bytes / k_avg_s / 1e9
+    return dict(solver=s, body=body, info=info, mlups=mlups, wall=wall, k_avg_ms=k_avg_s * 1e3, fluid=fluid,
+                bulk_cells=bulk_cells, wall_cells=wall_cells, k_bytes=k_bytes, achieved_gbs=achieved,
+                unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes)
+
+
+def copy_probe(nbytes=1 << 30, iters=20):
+    from windtunnel_vis_amd import _native as N
+    lib = N.load()
+    g = N.C.c_double()
+    N.check(lib.wt_copy_probe(int(nbytes), int(iters), N.C.byref(g)))
+    return g.value
 
 
 def physics(s, body, info, u, total=5000, chunk=250):
@@ -131,12 +141,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU-baseline sample steps (0 = auto)")
     ap.add_argument("--north-star", action="store_true", help="also time 512x256x256")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for counter collection: copy probe + the main workload only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     dims = tuple(int(v) for v in args.dims.split(","))
-    r = run_gpu(dims, args.steps, args.warmup, args.precision)
+    if args.profile_run:
+        args.no_physics = args.no_cpu = True
+        args.north_star = False
+    copy_gbs = copy_probe(256 << 20, 3) if args.profile_run else copy_probe()
+    r = run_gpu(dims, args.steps, args.warmup, args.precision, xcd=1)
+    rx = None
+    if not args.profile_run:
+        rx = run_gpu(dims, max(args.steps // 4, 200), args.warmup, args.precision, xcd=0)
+        rx["solver"].destroy()
     out = {
         "metric": METRIC, "value": round(r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 5), "higher_is_better": True,
@@ -148,11 +168,14 @@ def main():
     frac = r["achieved_gbs"] / HBM_PEAK_GBS
     out["roofline"] = {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(frac, 4),
-                       "traffic": traffic_from_profiles("pmc_bulk_d3q19_f16_%dx%dx%d.txt" % dims),
-                       "kernel": "k_bulk<D3Q19,SF16>", "bytes_per_launch": r["bulk_bytes"],
-                       "avg_launch_ms": round(r["bulk_avg_ms"], 5), "cells_per_launch": r["bulk_cells"]}
+                       "traffic": traffic_from_profiles("pmc_step_d3q19_f16_%dx%dx%d.txt" % dims),
+                       "kernel": "k_step<D3Q19,SF16,2>", "bytes_per_launch": r["k_bytes"],
+                       "avg_launch_ms": round(r["k_avg_ms"], 5), "fluid_cells_per_launch": r["fluid"],
+                       "measured_copy_gbs": round(copy_gbs, 1)}
     out["step_roofline_frac"] = round(r["mlups"] * 1e6 * 2 * r["q"] * r["sbytes"] / (HBM_PEAK_GBS * 1e9), 4)
-    out["wall_kernel"] = {"avg_launch_ms": round(r["wall_avg_ms"], 5), "cells": r["wall_cells"]}
+    out["work_split"] = {"bulk_fluid_cells": r["bulk_cells"], "wall_cells": r["wall_cells"]}
+    if rx:
+        out["xcd_remap_off_mlups"] = round(rx["mlups"], 1)
     out["unstable"] = bool(r["unstable"])
     if not args.no_physics:
         ph = physics(r["solver"], r["body"], r["info"], 0.08)
@@ -164,7 +187,8 @@ def main():
         rb = run_gpu(big, max(args.steps // 4, 200), args.warmup, args.precision)
         out["north_star_512x256x256"] = {"mlups": round(rb["mlups"], 1),
                                           "step_roofline_frac": round(rb["mlups"] * 1e6 * 2 * rb["q"] * rb["sbytes"] / (HBM_PEAK_GBS * 1e9), 4),
-                                          "bulk_frac": round(rb["achieved_gbs"] / HBM_PEAK_GBS, 4)}
+                                          "kernel_frac": round(rb["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                                          "avg_launch_ms": round(rb["k_avg_ms"], 5)}
         rb["solver"].destroy()
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)

@@ -208,15 +208,25 @@ int wt_read_macro(wt_solver* s, float* vel4, float* mean_a4, float* mean_b);
 
 /* ---- profiling -------------------------------------------------------------------------- */
 
-/* When enabled, every step records HIP events around its bulk kernel on the handle's stream.
- * wt_profile_read synchronises and returns the summed bulk-kernel time (ms), the number of bulk
- * launches timed, and the summed wall-kernel time; then clears the accumulators. */
+/* When enabled, every step records HIP events around its step kernel (one launch per step) on the
+ * handle's stream. wt_profile_read synchronises and returns the summed kernel time (ms), the
+ * number of launches timed, and 0 in the third slot (kept for hosts that read it); then clears the
+ * accumulators. */
 int wt_profile_enable(wt_solver* s, int32_t on);
 int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* boundary_ms);
 
-/* Number of fluid cells the bulk kernel updates per step and the number of cells the wall kernel
- * updates (cells next to the body). */
+/* Number of fluid cells the bulk part of the step kernel updates and the number of cells its wall
+ * part updates (cells next to the body). */
 int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cells);
+
+/* HBM calibration (no reference counterpart): copies `bytes` between two fresh device buffers
+ * `iters` times with a plain streaming kernel and returns read + write GB/s. */
+int wt_copy_probe(int64_t bytes, int32_t iters, double* gbs);
+
+/* Launch tuning knobs (no reference counterpart; set by the autotuner). */
+enum { WT_TUNE_XCD_REMAP = 0 };
+int wt_set_tuning(wt_solver* s, int32_t key, int32_t value);
+int wt_get_tuning(const wt_solver* s, int32_t key, int32_t* value);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,9 @@ SIGNATURES = {
     "wt_profile_enable": (C.c_int, [_P, C.c_int32]),
     "wt_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "wt_work_split": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "wt_copy_probe": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "wt_set_tuning": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "wt_get_tuning": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
 }
 
 _lib = None

@@ -1,5 +1,7 @@
 // Lattice-Boltzmann kernels for gfx950: init, cell classification, the bulk fused
 // pull-stream + collide kernel, the wall kernel, and the macro / momentum-exchange reduction.
+#include <algorithm>
+
 #include "lbm.hpp"
 #include "launch.hpp"
 
@@ -188,16 +190,11 @@ __global__ __launch_bounds__(BLOCK) void k_wallprep(const uint32_t* __restrict__
 // HBM traffic per cell: Q loads + Q stores of S::BYTES (the algorithmic 2 Q s) + 1 kind byte.
 // ---------------------------------------------------------------------------------------------
 template <class L, class S, int CPT>
-__global__ __launch_bounds__(BLOCK) void k_bulk(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                                const uint8_t* __restrict__ kind, KParams P) {
+__device__ __forceinline__ void bulk_cells(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                           const uint8_t* __restrict__ kind, const KParams& P, uint32_t row, int x0) {
   using T = typename S::T;
   constexpr int Q = L::Q;
   constexpr bool D3 = L::DIM == 3;
-  const uint32_t ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const uint32_t row = blockIdx.x * 4u + ty;
-  const uint32_t nrows = (uint32_t)P.ny * (uint32_t)P.nz;
-  if (row >= nrows) return;
-  const int x0 = (int)(blockIdx.y * 64u + threadIdx.x) * CPT;
   const int NX = P.nx, NY = P.ny, NZ = P.nz;
   if (x0 >= NX) return;
   const int64_t n = P.n;
@@ -291,22 +288,49 @@ __global__ __launch_bounds__(BLOCK) void k_bulk(const typename S::T* __restrict_
   }
 }
 
-// Wall kernel: one thread per listed cell (cells next to the body), the reference's full branch
-// logic with precomputed solid masks.
+// Wall cells: one lane per listed cell (cells next to the body), the reference's full branch
+// logic with precomputed solid masks and link records.
 template <class L, class S>
-__global__ __launch_bounds__(BLOCK) void k_wall(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                                const uint8_t* __restrict__ kind, const uint32_t* __restrict__ list,
-                                                const uint32_t* __restrict__ masks, const WallRec* __restrict__ recs,
-                                                int64_t count, KParams P, KWheels W) {
-  const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= count) return;
-  const int64_t c = list[t];
+__device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                          const uint8_t* __restrict__ kind, const StepArgs& A, const KParams& P,
+                                          const KWheels& W, int64_t t) {
+  if (t >= A.nwall) return;
+  const int64_t c = A.wlist[t];
   int ix, iy, iz;
   coords(c, P, ix, iy, iz);
   float f[L::Q];
-  update_wall<L, S>(fin, kind[c], masks[t], recs[t], P, W, c, ix, iy, iz, f);
+  update_wall<L, S>(fin, kind[c], A.wmask[t], A.wrec[t], P, W, c, ix, iy, iz, f);
 #pragma unroll
   for (int i = 0; i < L::Q; i++) S::st_dev(fout, (int64_t)i * P.n + c, L::w(i), f[i]);
+}
+
+// One lattice step in one launch. The first A.nwb blocks update the wall cells (long dependency
+// chains, few lanes: started first so their latency hides under the bulk work); every other block
+// is 4 waves of the bulk update, waves enumerated (row, chunk) with the chunk fastest so a block
+// covers contiguous memory. With A.xcd set, bulk blocks are remapped so that each XCD (blocks are
+// dealt round-robin to the 8 XCDs) walks one contiguous eighth of the grid, keeping the cache
+// lines shared by neighbouring chunks inside one L2.
+template <class L, class S, int CPT>
+__global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                const uint8_t* __restrict__ kind, KParams P, KWheels W, StepArgs A) {
+  uint32_t b = blockIdx.x;
+  if (b < A.nwb) {
+    wall_cell<L, S>(fin, fout, kind, A, P, W, (int64_t)b * BLOCK + threadIdx.x);
+    return;
+  }
+  b -= A.nwb;
+  if (A.xcd) {
+    const uint32_t per = (A.nbb + 7u) / 8u;
+    const uint32_t lb = (b % 8u) * per + b / 8u;
+    if (lb >= A.nbb) return;
+    b = lb;
+  }
+  const uint32_t wave = b * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (wave >= A.nwaves) return;
+  const uint32_t row = wave / A.wpr;
+  const uint32_t chunk = wave - row * A.wpr;
+  const int x0 = (int)(chunk * 64u + (threadIdx.x & 63u)) * CPT;
+  bulk_cells<L, S, CPT>(fin, fout, kind, P, row, x0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -497,6 +521,17 @@ __global__ __launch_bounds__(BLOCK) void k_pack(const float* __restrict__ in, ty
   f[e] = S::enc(in[e], w);
 }
 
+// Plain streaming copy (16-byte vectors, grid-stride): the HBM-bandwidth calibration kernel.
+__global__ __launch_bounds__(BLOCK) void k_copy(const uint4* __restrict__ a, uint4* __restrict__ b, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n16; i += (int64_t)gridDim.x * BLOCK) b[i] = a[i];
+}
+
+void launch_copy(hipStream_t st, const void* a, void* b, int64_t bytes) {
+  const int64_t n16 = bytes / 16;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n16 + BLOCK - 1) / BLOCK, 256 * 64);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(BLOCK), 0, st, (const uint4*)a, (uint4*)b, n16);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_clear_stats(float4* __restrict__ meanA, float* __restrict__ meanB, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
@@ -556,27 +591,32 @@ void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* m
     hipLaunchKernelGGL((k_wallprep<D3Q19>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, c.flags, c.sdf, list, masks, recs, count, c.P);
 }
 
-void launch_bulk(const LaunchCtx& c, const void* fin, void* fout) {
+StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, const WallRec* recs, int64_t count, int xcd) {
+  StepArgs A;
   const int cpt = cells_per_thread(c.P);
-  const unsigned rows = (unsigned)((int64_t)c.P.ny * c.P.nz);
-  const dim3 grid((rows + 3) / 4, (unsigned)((c.P.nx + 64 * cpt - 1) / (64 * cpt)));
-  const dim3 block(64, 4);
+  A.wlist = list;
+  A.wmask = masks;
+  A.wrec = recs;
+  A.nwall = count;
+  A.nwb = (uint32_t)(((count + BLOCK - 1) / BLOCK + 7) / 8 * 8);  // a multiple of 8 keeps bulk blocks XCD-aligned
+  A.wpr = (uint32_t)((c.P.nx + 64 * cpt - 1) / (64 * cpt));
+  A.nwaves = (uint32_t)((int64_t)c.P.ny * c.P.nz * A.wpr);
+  A.nbb = (A.nwaves + 3u) / 4u;
+  A.xcd = xcd;
+  return A;
+}
+
+void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A) {
+  const int cpt = cells_per_thread(c.P);
+  const dim3 grid(A.nwb + (A.xcd ? (A.nbb + 7u) / 8u * 8u : A.nbb));
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
     if (cpt == 2)
-      hipLaunchKernelGGL((k_bulk<L, S, 2>), grid, block, 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind, c.P);
+      hipLaunchKernelGGL((k_step<L, S, 2>), grid, dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind,
+                         c.P, *c.W, A);
     else
-      hipLaunchKernelGGL((k_bulk<L, S, 1>), grid, block, 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind, c.P);
-  });
-}
-
-void launch_wall(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, const uint32_t* masks, const WallRec* recs,
-                 int64_t count) {
-  if (count <= 0) return;
-  dispatch(c.P.q, c.prec, [&](auto l, auto s) {
-    using L = decltype(l); using S = decltype(s);
-    hipLaunchKernelGGL((k_wall<L, S>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, (const typename S::T*)fin,
-                       (typename S::T*)fout, c.kind, list, masks, recs, count, c.P, *c.W);
+      hipLaunchKernelGGL((k_step<L, S, 1>), grid, dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind,
+                         c.P, *c.W, A);
   });
 }
 

@@ -167,7 +167,7 @@ __device__ inline int dim_of(const MeshCtx& C, int a) { return a == 0 ? C.nx : (
 __device__ inline int64_t cell_of(const MeshCtx& C, int x, int y, int z) { return (int64_t)x + (int64_t)C.nx * ((int64_t)y + (int64_t)C.ny * z); }
 __device__ inline float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ inline float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ inline float len3s(float3 a) { return __fsqrt_rn(dot3(a, a)); }
+__device__ inline float len3s(float3 a) { return __builtin_sqrtf(dot3(a, a)); }
 __device__ inline float3 cross3(float3 a, float3 b) {
   return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(64) void k_surface(const float* __restrict__ pos, c
       float tt = tol;
       if (is2D) {
         // the 2D grid is one plane: compare the in-plane distance of the section line
-        const float n2 = __fsqrt_rn(nrm.x * nrm.x + nrm.y * nrm.y);
+        const float n2 = __builtin_sqrtf(nrm.x * nrm.x + nrm.y * nrm.y);
         if (n2 < 1e-4f) continue;
         ctr.z = p.z;
         tt = 0.5f * fmaxf(fabsf(nrm.x), fabsf(nrm.y));

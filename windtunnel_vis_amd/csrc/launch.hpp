@@ -17,14 +17,14 @@ struct LaunchCtx {
 void launch_init(const LaunchCtx& c, void* f);
 void launch_classify(const LaunchCtx& c, uint32_t* list, uint32_t* masks, unsigned long long* counts);
 void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, WallRec* recs, int64_t count);
-void launch_bulk(const LaunchCtx& c, const void* fin, void* fout);
-void launch_wall(const LaunchCtx& c, const void* fin, void* fout, const uint32_t* list, const uint32_t* masks, const WallRec* recs,
-                 int64_t count);
+StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, const WallRec* recs, int64_t count, int xcd);
+void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A);
 int cells_per_thread(const KParams& P);
 int64_t macro_blocks(int64_t n);
 void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, float* meanB, double* partials, double* result);
 void launch_unpack(const LaunchCtx& c, const void* f, float* out);
 void launch_pack(const LaunchCtx& c, const float* in, void* f);
+void launch_copy(hipStream_t st, const void* a, void* b, int64_t bytes);
 void launch_clear_stats(const LaunchCtx& c, void* meanA, float* meanB);
 
 }  // namespace wt

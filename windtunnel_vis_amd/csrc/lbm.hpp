@@ -201,6 +201,20 @@ struct WallRec {
   uint8_t tag[20];   // wheel index + 1 of the solid cell a link points into (0: not a wheel)
 };
 
+// Launch layout of one step (see k_step): wall blocks first, then bulk blocks of 4 waves.
+struct StepArgs {
+  const uint32_t* wlist;
+  const uint32_t* wmask;
+  const WallRec* wrec;
+  int64_t nwall;
+  uint32_t nwb;     // wall blocks (multiple of 8)
+  uint32_t wpr;     // waves per x-row
+  uint32_t nwaves;  // bulk waves
+  uint32_t nbb;     // bulk blocks
+  int32_t xcd;      // XCD-aware remap of bulk blocks
+  int32_t pad;
+};
+
 // wgslLbm.ts:21-28
 __device__ inline float3 wheel_vel(const KWheels& W, const KParams& P, uint32_t k, float3 p) {
   float4 a = W.w[2 * k], b = W.w[2 * k + 1];

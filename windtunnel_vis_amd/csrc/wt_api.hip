@@ -64,7 +64,8 @@ struct wt_solver {
   int64_t bytes = 0;
   // profiling
   bool prof = false;
-  std::vector<hipEvent_t> ev;   // per step: bulk start, bulk stop (= wall start), -, wall stop
+  std::vector<hipEvent_t> ev;   // per step: start, stop of the step kernel
+  int xcd_remap = 1;
   size_t ev_used = 0;
   double acc_bulk = 0, acc_bnd = 0;
   int64_t acc_launches = 0;
@@ -172,26 +173,23 @@ int do_steps(wt_solver* s, int steps) {
     if (r) return r;
   }
   LaunchCtx c = s->ctx();
+  const StepArgs A = step_args(c, s->wlist, s->wmask, s->wrec, s->nwall, s->xcd_remap);
   for (int k = 0; k < steps; k++) {
     const void* fin = s->f[s->parity];
     void* fout = s->f[s->parity ^ 1];
     if (s->prof) {
-      if (s->ev_used + 4 > s->ev.size()) {
+      if (s->ev_used + 2 > s->ev.size()) {
         size_t old = s->ev.size();
         s->ev.resize(std::max<size_t>(64, old * 2));
         for (size_t i = old; i < s->ev.size(); i++) HIP_TRY(hipEventCreate(&s->ev[i]));
       }
       hipEvent_t* e = &s->ev[s->ev_used];
-      s->ev_used += 4;
+      s->ev_used += 2;
       HIP_TRY(hipEventRecord(e[0], s->stream));
-      launch_bulk(c, fin, fout);
+      launch_step(c, fin, fout, A);
       HIP_TRY(hipEventRecord(e[1], s->stream));
-      launch_wall(c, fin, fout, s->wlist, s->wmask, s->wrec, s->nwall);
-      HIP_TRY(hipEventRecord(e[3], s->stream));
-      e[2] = e[1];
     } else {
-      launch_bulk(c, fin, fout);
-      launch_wall(c, fin, fout, s->wlist, s->wmask, s->wrec, s->nwall);
+      launch_step(c, fin, fout, A);
     }
     s->parity ^= 1;
   }
@@ -517,12 +515,10 @@ int wt_profile_enable(wt_solver* s, int32_t on) {
 int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* boundary_ms) {
   if (!s) return fail(WT_E_ARG, "null argument");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  for (size_t i = 0; i + 4 <= s->ev_used; i += 4) {
-    float a = 0, b = 0;
+  for (size_t i = 0; i + 2 <= s->ev_used; i += 2) {
+    float a = 0;
     HIP_TRY(hipEventElapsedTime(&a, s->ev[i], s->ev[i + 1]));
-    HIP_TRY(hipEventElapsedTime(&b, s->ev[i + 2], s->ev[i + 3]));
     s->acc_bulk += a;
-    s->acc_bnd += b;
     s->acc_launches++;
   }
   s->ev_used = 0;
@@ -532,6 +528,49 @@ int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* bo
   s->acc_bulk = s->acc_bnd = 0;
   s->acc_launches = 0;
   return WT_OK;
+}
+
+int wt_copy_probe(int64_t bytes, int32_t iters, double* gbs) {
+  if (bytes < 16 || iters < 1 || !gbs) return fail(WT_E_ARG, "bad copy probe arguments");
+  bytes &= ~(int64_t)15;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    if (a) hipFree(a);
+    hipStreamDestroy(st);
+    return fail(WT_E_NOMEM, "copy probe allocation failed");
+  }
+  HIP_TRY(hipMemsetAsync(a, 1, bytes, st));
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  launch_copy(st, a, b, bytes);  // warm-up
+  HIP_TRY(hipEventRecord(e0, st));
+  for (int k = 0; k < iters; k++) launch_copy(st, a, b, bytes);
+  HIP_TRY(hipEventRecord(e1, st));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *gbs = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  hipStreamDestroy(st);
+  return WT_OK;
+}
+
+int wt_set_tuning(wt_solver* s, int32_t key, int32_t value) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (key == WT_TUNE_XCD_REMAP) { s->xcd_remap = value ? 1 : 0; return WT_OK; }
+  return fail(WT_E_ARG, "unknown tuning key");
+}
+
+int wt_get_tuning(const wt_solver* s, int32_t key, int32_t* value) {
+  if (!s || !value) return fail(WT_E_ARG, "null argument");
+  if (key == WT_TUNE_XCD_REMAP) { *value = s->xcd_remap; return WT_OK; }
+  return fail(WT_E_ARG, "unknown tuning key");
 }
 
 int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cells) {

@@ -237,6 +237,14 @@ class SolverGPU:
         N.check(self._lib.wt_profile_read(self._h, N.C.byref(b), N.C.byref(k), N.C.byref(bd)))
         return b.value, k.value, bd.value
 
+    def setTuning(self, key, value):
+        N.check(self._lib.wt_set_tuning(self._h, int(key), int(value)))
+
+    def getTuning(self, key):
+        v = N.C.c_int32()
+        N.check(self._lib.wt_get_tuning(self._h, int(key), N.C.byref(v)))
+        return v.value
+
     def workSplit(self):
         a = N.C.c_int64()
         b = N.C.c_int64()
