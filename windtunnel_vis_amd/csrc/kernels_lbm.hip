@@ -232,12 +232,13 @@ __device__ __forceinline__ void bulk_cells(const typename S::T* __restrict__ fin
     src[i] = fin + (int64_t)d * n + ((int64_t)sy + (int64_t)NY * sz) * NX + xs;
     xlim[i] = (grnd || cx >= 0) ? NX - 1 : NX - 2;
   }
-  // ---- 2. all loads in flight together ----
+  // ---- 2. all loads in flight together (scalar base + 32-bit lane byte offset) ----
+  const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
   float fa[Q], fb[Q];
   if constexpr (CPT == 2) {
     typename S::Word raw[Q];
 #pragma unroll
-    for (int i = 0; i < Q; i++) raw[i] = S::ld_word(src[i] + x0);
+    for (int i = 0; i < Q; i++) raw[i] = S::ld_word((const T*)((const char*)src[i] + lane_off));
 #pragma unroll
     for (int i = 0; i < Q; i++) S::unpack_word(raw[i], L::w(i), fa[i], fb[i]);
   } else {
@@ -282,7 +283,7 @@ __device__ __forceinline__ void bulk_cells(const typename S::T* __restrict__ fin
   if constexpr (CPT == 2) collide<L>(fb, (float)(x0 + 1), gwall, gwall, P);
 #pragma unroll
   for (int i = 0; i < Q; i++) {
-    T* o = fout + (int64_t)i * n + c0;
+    T* o = (T*)((char*)(fout + uniform_i64((int64_t)i * n + rowoff)) + lane_off);
     if constexpr (CPT == 2) S::st2_dev(o, L::w(i), fa[i], fb[i]);
     else S::st_dev(o, 0, L::w(i), fa[i]);
   }
@@ -522,14 +523,25 @@ __global__ __launch_bounds__(BLOCK) void k_pack(const float* __restrict__ in, ty
 }
 
 // Plain streaming copy (16-byte vectors, grid-stride): the HBM-bandwidth calibration kernel.
-__global__ __launch_bounds__(BLOCK) void k_copy(const uint4* __restrict__ a, uint4* __restrict__ b, int64_t n16) {
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n16; i += (int64_t)gridDim.x * BLOCK) b[i] = a[i];
+// Each lane moves 4 independent 16-byte vectors per iteration so enough bytes are in flight.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(BLOCK) void k_copy(const u32x4* __restrict__ a, u32x4* __restrict__ b, int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const u32x4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+    __builtin_nontemporal_store(v0, &b[i]);
+    __builtin_nontemporal_store(v1, &b[i + stride]);
+    __builtin_nontemporal_store(v2, &b[i + 2 * stride]);
+    __builtin_nontemporal_store(v3, &b[i + 3 * stride]);
+  }
+  for (; i < n16; i += stride) __builtin_nontemporal_store(a[i], &b[i]);
 }
 
 void launch_copy(hipStream_t st, const void* a, void* b, int64_t bytes) {
   const int64_t n16 = bytes / 16;
-  const unsigned blocks = (unsigned)std::min<int64_t>((n16 + BLOCK - 1) / BLOCK, 256 * 64);
-  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(BLOCK), 0, st, (const uint4*)a, (uint4*)b, n16);
+  const unsigned blocks = (unsigned)std::min<int64_t>((n16 + BLOCK - 1) / BLOCK, 256 * 16);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(BLOCK), 0, st, (const u32x4*)a, (u32x4*)b, n16);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_clear_stats(float4* __restrict__ meanA, float* __restrict__ meanB, int64_t n) {

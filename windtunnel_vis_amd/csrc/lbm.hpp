@@ -238,6 +238,14 @@ __device__ inline float link_q(const uint32_t* sdf, const KParams& P, int64_t fl
 
 __device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// A wave-uniform 64-bit offset kept in scalar registers, so that `pointer argument + uniform
+// offset + 32-bit lane offset` selects the SGPR-base / VGPR-offset form of the memory instruction.
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Inlet equilibrium deviation w_i (3 c_x U + 4.5 c_x^2 U^2 - 1.5 U^2) (rho_in = 1), wgslLbm.ts:146.
 template <class L>
 __device__ inline float inlet_dev(int i, float U) {
@@ -251,6 +259,17 @@ __device__ inline float inlet_dev(int i, float U) {
 // are formed per opposite-direction pair (i, opp i): the even part (mass, stress) and the odd part
 // (momentum) once per pair. Algebraically identical to the reference's per-direction form.
 // ---------------------------------------------------------------------------------------------
+// Sum of c * x over three components with compile-time-known small integer coefficients: zero
+// terms are dropped and +-1 terms become adds / subtracts. Accumulation starts from -0.0, the exact
+// additive identity, so nothing is left for IEEE rules to keep (0 * x and x + 0.0 are not foldable).
+__device__ __forceinline__ float csum(int a, float x, int b, float y, int c, float z) {
+  float r = -0.0f;
+  if (a == 1) r += x; else if (a == -1) r -= x; else if (a != 0) r += (float)a * x;
+  if (b == 1) r += y; else if (b == -1) r -= y; else if (b != 0) r += (float)b * y;
+  if (c == 1) r += z; else if (c == -1) r -= z; else if (c != 0) r += (float)c * z;
+  return r;
+}
+
 template <class L>
 __device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall, bool gwall, const KParams& P) {
   constexpr int Q = L::Q;
@@ -266,7 +285,7 @@ __device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall,
     a[i] = d[i] - d[o];
     drho += s[i];
   }
-  float jx = 0.f, jy = 0.f, jz = 0.f, sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+  float jx = -0.0f, jy = -0.0f, jz = -0.0f, sxx = -0.0f, syy = -0.0f, szz = -0.0f, sxy = -0.0f, sxz = -0.0f, syz = -0.0f;
 #pragma unroll
   for (int i = 1; i < Q; i++) {
     const int o = opp<L>(i);
@@ -294,25 +313,30 @@ __device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall,
   const float pxy = sxy - jx * uy;
   const float pxz = D3 ? sxz - jx * uz : 0.0f;
   const float pyz = D3 ? syz - jy * uz : 0.0f;
-  const float usq = ux * ux + uy * uy + uz * uz;
+  const float usq = D3 ? ux * ux + uy * uy + uz * uz : ux * ux + uy * uy;
   const float sp = __builtin_amdgcn_fmed3f((xpos - P.spongeStart) * P.invSponge, 0.0f, 1.0f);
   float t0 = P.tau0 + 3.0f * P.spongeNu * sp * sp;
   if (gwall) t0 = fmaxf(t0, P.tauWall);
   float tau = t0;
   if (P.smag > 0.0f) {
-    const float piN = __builtin_sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
-    tau = 0.5f * (t0 + __builtin_sqrtf(t0 * t0 + P.smag * piN * irho));
+    const float pn2 = D3 ? pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz)
+                         : pxx * pxx + pyy * pyy + 2.0f * (pxy * pxy);
+    // hardware square roots (1 ulp): tau only scales the relaxation
+    const float piN = __builtin_amdgcn_sqrtf(pn2);
+    tau = 0.5f * (t0 + __builtin_amdgcn_sqrtf(t0 * t0 + P.smag * piN * irho));
   }
   const float omega = fast_rcp(tau);
   const float om1 = 1.0f - omega;
   const float si = __builtin_amdgcn_fmed3f(1.0f - xpos * P.invSpongeIn, 0.0f, 1.0f);
   const float sIn = P.spongeIn > 0.0f ? 0.08f * si * si : 0.0f;
   const float rho45 = 4.5f * rho;
+  // (p_aa + p_bb) shared by the diagonal directions of each plane
+  const float pxy_d = pxx + pyy, pxz_d = pxx + pzz, pyz_d = pyy + pzz;
   float out[Q];
   if (!(wall || P.collision == 1)) {
     // regularized: f_i - w_i = w_i (E + 4.5 rho cu^2 + k1 Q_i:Pi) +- 3 w_i c_i.j
     const float k1 = 4.5f * om1;
-    const float trP = (pxx + pyy + pzz) * third;
+    const float trP = D3 ? (pxx + pyy + pzz) * third : (pxx + pyy) * third;
     const float E = drho - 1.5f * rho * usq - k1 * trP;
     out[0] = L::w(0) * E;
 #pragma unroll
@@ -320,10 +344,14 @@ __device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall,
       const int o = opp<L>(i);
       if (o < i) continue;
       const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-      const float cu = cx * ux + cy * uy + cz * uz;
-      const float cj = cx * jx + cy * jy + cz * jz;
-      const float qp = (float)(cx * cx) * pxx + (float)(cy * cy) * pyy + (float)(cz * cz) * pzz +
-                       (float)(2 * cx * cy) * pxy + (float)(2 * cx * cz) * pxz + (float)(2 * cy * cz) * pyz;
+      const float cu = csum(cx, ux, cy, uy, cz, uz);
+      const float cj = csum(cx, jx, cy, jy, cz, jz);
+      // Q_i : Pi for |c| = 1 (one diagonal entry) or |c|^2 = 2 (two diagonal entries + 2 off-diagonal)
+      float qp;
+      if (cx && cy) qp = pxy_d + (float)(2 * cx * cy) * pxy;
+      else if (cx && cz) qp = pxz_d + (float)(2 * cx * cz) * pxz;
+      else if (cy && cz) qp = pyz_d + (float)(2 * cy * cz) * pyz;
+      else qp = cx ? pxx : (cy ? pyy : pzz);
       const float ev = L::w(i) * (E + rho45 * cu * cu + k1 * qp);
       const float od = (3.0f * L::w(i)) * cj;
       out[i] = ev + od;
@@ -338,12 +366,12 @@ __device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall,
       const int o = opp<L>(i);
       if (o < i) continue;
       const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-      const float cu = cx * ux + cy * uy + cz * uz;
-      const float cj = cx * jx + cy * jy + cz * jz;
+      const float cu = csum(cx, ux, cy, uy, cz, uz);
+      const float cj = csum(cx, jx, cy, jy, cz, jz);
       const float ev = L::w(i) * (E + rho45 * cu * cu);
       const float od = (3.0f * L::w(i)) * cj;
-      out[i] = omega * (ev + od) + om1 * d[i];
-      out[o] = omega * (ev - od) + om1 * d[o];
+      out[i] = (ev + od) + om1 * (d[i] - (ev + od));
+      out[o] = (ev - od) + om1 * (d[o] - (ev - od));
     }
   }
   if (sIn > 0.0f) {  // inlet absorbing layer: mix toward feq(1, U), wgslLbm.ts:220, :238
