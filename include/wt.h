@@ -208,10 +208,10 @@ int wt_read_macro(wt_solver* s, float* vel4, float* mean_a4, float* mean_b);
 
 /* ---- profiling -------------------------------------------------------------------------- */
 
-/* When enabled, every step records HIP events around its step kernel (one launch per step) on the
- * handle's stream. wt_profile_read synchronises and returns the summed kernel time (ms), the
- * number of launches timed, and 0 in the third slot (kept for hosts that read it); then clears the
- * accumulators. */
+/* When enabled, every step records HIP events around its step kernel (one launch per step: wall
+ * cells and bulk cells) on the handle's stream. wt_profile_read synchronises and returns the summed
+ * kernel time (ms), the number of launches timed, and 0 in the third slot (kept for hosts that read
+ * it); then clears the accumulators. */
 int wt_profile_enable(wt_solver* s, int32_t on);
 int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* boundary_ms);
 
@@ -223,7 +223,8 @@ int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cel
  * `iters` times with a plain streaming kernel and returns read + write GB/s. */
 int wt_copy_probe(int64_t bytes, int32_t iters, double* gbs);
 
-/* Launch tuning knobs (no reference counterpart; set by the autotuner). */
+/* Launch tuning knobs (no reference counterpart; set by the autotuner).
+ * WT_TUNE_XCD_REMAP: 1 places each eighth of the bulk items on one XCD (default 1). */
 enum { WT_TUNE_XCD_REMAP = 0 };
 int wt_set_tuning(wt_solver* s, int32_t key, int32_t value);
 int wt_get_tuning(const wt_solver* s, int32_t key, int32_t* value);

@@ -84,33 +84,104 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const uint32_t* __restrict__
 // Groups of CPT cells along x (the bulk kernel's unit) that contain a WALL cell, or mix solid and
 // fluid, go to the wall kernel; their cells are listed with their solid masks.
 // counts[0] = listed cells, counts[1] = bulk fluid cells, counts[2] = bulk solid cells.
-// Per-link data of a wall cell, fixed by the geometry: the Bouzidi fraction q of every solid link
-// (with the reference's fallback to q = 1/2 when the second fluid node is missing folded in,
-// wgslLbm.ts:172) and the wheel tag of the solid cell it points into.
+// The pulled populations of a wall cell as links (see WallLink), following the reference's branch
+// order for a pull (wgslLbm.ts:139-175): inlet, ground (bounce-back + belt), y-z edge mirror,
+// y mirror, z periodic / mirror, solid neighbour (halfway or Bouzidi, with the fallback to q = 1/2
+// when the second fluid node is missing, wgslLbm.ts:172; rotating-wheel velocity at the link
+// midpoint, wgslLbm.ts:21-28), plain pull; then the per-cell wheel mass fix (wgslLbm.ts:289-291)
+// folded into the constant terms. Depends on flags, distances, wheels and the parameters.
 template <class L>
-__device__ WallRec wall_record(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sdf, int64_t c, int ix, int iy,
-                               int iz, uint32_t m, KParams P) {
-  P.interp = 1;  // the record holds the interpolated q; halfway walls select 1/2 at step time
-  WallRec R;
+__device__ void wall_links(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sdf, uint8_t kd, int64_t c,
+                           int ix, int iy, int iz, uint32_t smask, const KParams& P, const KWheels& W,
+                           WallLink* __restrict__ out, int64_t cap, uint32_t& meta) {
+  constexpr int Q = L::Q, DIM = L::DIM;
+  const int64_t n = P.n;
+  const int NX = P.nx, NY = P.ny, NZ = P.nz;
+  auto cell = [&](int x, int y, int z) -> int64_t { return (int64_t)x + (int64_t)NX * ((int64_t)y + (int64_t)NY * z); };
+  const float U = P.uin;
+  bool wall = false, gwall = false;
+  float mvSum = 0.f, mvW = 0.f;
+  float mv[Q];
+  const float belt = (P.ground == 2 && !(kd & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
 #pragma unroll
-  for (int i = 0; i < 20; i++) { R.q[i < 19 ? i : 18] = 0.5f; R.tag[i] = 0; }
-#pragma unroll
-  for (int i = 1; i < L::Q; i++) {
-    if (!(m & (1u << i))) continue;
+  for (int i = 1; i < Q; i++) {
     const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-    const int sx = cx < 0 ? min(ix - cx, P.nx - 1) : ix - cx;
-    const int64_t sc = (int64_t)sx + (int64_t)P.nx * ((int64_t)(iy - cy) + (int64_t)P.ny * (L::DIM == 3 ? iz - cz : 0));
-    float q = link_q(sdf, P, c, sc);
-    if (q < 0.5f) {
-      const int ax = ix + cx, ay = iy + cy, az = L::DIM == 3 ? iz + cz : 0;
-      const bool ok = ax >= 0 && ay >= 0 && az >= 0 && ax < P.nx && ay < P.ny && az < P.nz &&
-                      flags[(int64_t)ax + (int64_t)P.nx * ((int64_t)ay + (int64_t)P.ny * az)] == 0u;
-      if (!ok) q = 0.5f;
+    const float w = L::w(i);
+    int sx = ix - cx, sy = iy - cy, sz = DIM == 3 ? iz - cz : 0;
+    if (cx < 0) sx = min(sx, NX - 1);
+    const bool yOut = cy > 0 ? sy < 0 : (cy < 0 ? sy >= NY : false);
+    const bool zOut = DIM == 3 ? (cz > 0 ? sz < 0 : (cz < 0 ? sz >= NZ : false)) : false;
+    WallLink k;
+    k.e0 = (int64_t)i * n + c;
+    k.a0 = 1.0f;
+    k.a1 = 0.0f;
+    k.b = 0.0f;
+    k.pad = 0u;
+    mv[i] = -1.0f;  // marks links without a wheel term
+    if (cx > 0 && sx < 0) {
+      k.a0 = 0.0f;
+      k.b = inlet_dev<L>(i, U);
+    } else if (cy > 0 && yOut && P.ground != 0) {
+      wall = true; gwall = true;
+      k.e0 = (int64_t)opp<L>(i) * n + c;
+      if (cx != 0) k.b = belt * ((6.0f * w * cx) * P.uGround);
+    } else {
+      const int sxc = min(max(sx, 0), NX - 1);
+      if (DIM == 3 && yOut && zOut) {
+        k.e0 = (int64_t)mirYZ<L>(i) * n + cell(sxc, iy, iz);
+      } else if (yOut) {
+        k.e0 = (int64_t)mirY<L>(i) * n + cell(sxc, iy, DIM == 3 ? min(max(sz, 0), NZ - 1) : 0);
+      } else if (DIM == 3 && zOut) {
+        if (P.sideMode == 1) k.e0 = (int64_t)i * n + cell(sxc, sy, (sz + NZ) % NZ);
+        else k.e0 = (int64_t)mirZ<L>(i) * n + cell(sxc, sy, iz);
+      } else {
+        const int64_t sc = cell(sxc, sy, sz);
+        if (smask & (1u << i)) {
+          wall = true;
+          float q = 0.5f;
+          if (P.interp) {
+            q = link_q(sdf, P, c, sc);
+            if (q < 0.5f) {
+              const int ax = ix + cx, ay = iy + cy, az = DIM == 3 ? iz + cz : 0;
+              const bool ok = ax >= 0 && ay >= 0 && az >= 0 && ax < NX && ay < NY && az < NZ && flags[cell(ax, ay, az)] == 0u;
+              if (!ok) q = 0.5f;
+            }
+          }
+          k.e0 = (int64_t)opp<L>(i) * n + c;
+          k.e1 = k.e0;
+          if (q < 0.5f) {
+            k.a0 = 2.0f * q;
+            k.a1 = 1.0f - 2.0f * q;
+            k.e1 = (int64_t)opp<L>(i) * n + cell(ix + cx, iy + cy, DIM == 3 ? iz + cz : 0);
+          } else if (q > 0.5f) {
+            k.a0 = 0.5f / q;
+            k.a1 = 1.0f - 0.5f / q;
+            k.e1 = (int64_t)i * n + c;
+          }
+          const uint32_t tag = flags[sc] >> 8;
+          if (tag != 0u) {
+            const float3 p = make_float3(0.5f * (float)(ix + sxc) + 0.5f, 0.5f * (float)(iy + sy) + 0.5f,
+                                         0.5f * (float)(iz + sz) + 0.5f);
+            const float3 uw = wheel_vel(W, P, tag - 1u, p);
+            const float m = (6.0f * w) * ((float)cx * uw.x + (float)cy * uw.y + (float)cz * uw.z) * (q > 0.5f ? 0.5f / q : 1.0f);
+            mv[i] = m;
+            mvSum += m;
+            mvW += w;
+          }
+        } else {
+          k.e0 = (int64_t)i * n + sc;
+        }
+      }
     }
-    R.q[i] = q;
-    R.tag[i] = (uint8_t)(flags[sc] >> 8);
+    if (k.a1 == 0.0f) k.e1 = k.e0;
+    out[(int64_t)(i - 1) * cap] = k;
   }
-  return R;
+  // wheel mass fix: the mean injected momentum of the wheel links is taken back from them
+  const float corr = mvW > 0.0f ? mvSum / mvW : 0.0f;
+#pragma unroll
+  for (int i = 1; i < Q; i++)
+    if (mv[i] != -1.0f) out[(int64_t)(i - 1) * cap].b = mv[i] - L::w(i) * corr;
+  meta = (wall ? WM_WALL : 0u) | (gwall ? WM_GWALL : 0u) | ((uint32_t)ix << 8);
 }
 
 template <class L, int CPT>
@@ -167,54 +238,83 @@ __global__ __launch_bounds__(BLOCK) void k_collect(const uint8_t* __restrict__ k
   }
 }
 
-// Per-link records of the listed wall cells; rerun whenever flags or distances change.
+// Links of the listed wall cells; rerun whenever flags, distances, wheels or parameters change.
 template <class L>
 __global__ __launch_bounds__(BLOCK) void k_wallprep(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sdf,
-                                                    const uint32_t* __restrict__ list, const uint32_t* __restrict__ masks,
-                                                    WallRec* __restrict__ recs, int64_t count, KParams P) {
+                                                    const uint8_t* __restrict__ kind, const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ masks, WallLink* __restrict__ links,
+                                                    uint32_t* __restrict__ meta, int64_t count, int64_t cap, KParams P, KWheels W) {
   const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (t >= count) return;
   const int64_t c = list[t];
   int ix, iy, iz;
   coords(c, P, ix, iy, iz);
-  recs[t] = wall_record<L>(flags, sdf, c, ix, iy, iz, masks[t], P);
+  const uint8_t kd = kind[c];
+  if (kd & KIND_SOLID) {
+    meta[t] = WM_SOLID;
+    return;
+  }
+  uint32_t m = 0u;
+  wall_links<L>(flags, sdf, kd, c, ix, iy, iz, masks[t], P, W, links + t, cap, m);
+  meta[t] = m;
 }
 
 // ---------------------------------------------------------------------------------------------
-// Bulk kernel. Block = 4 waves; each wave owns a chunk of 64 * CPT cells of one x-row, so the row
+// Bulk update. An item is a chunk of 64 * CPT cells of one x-row, owned by one wave, so the row
 // (y, z) — and with it every face condition (ground, mirrors, periodic z, y-z edges) — is uniform
-// across the wave and is resolved with scalar arithmetic. For each direction the kernel picks the
-// source direction and source row once per wave, then every lane loads CPT consecutive cells
-// (x shifted by -c_x; unaligned dword loads for FP16/BF16 pairs). Inlet (x = 0) and outlet clamp
-// (x = nx - 1) are per-lane fix-ups. Groups holding a WALL cell are left to the wall kernel.
+// across the wave and is resolved with scalar arithmetic. For each direction the wave picks the
+// source direction and source row once, then every lane loads CPT consecutive cells (x shifted by
+// -c_x; unaligned dword loads for FP16/BF16 pairs). Inlet (x = 0) and outlet clamp (x = nx - 1)
+// are per-lane fix-ups. Groups holding a WALL cell are left to the wall path.
+// An item is split in two halves so that a wave can issue the loads of its next item before it
+// collides the current one (bulk_load / bulk_finish).
 // HBM traffic per cell: Q loads + Q stores of S::BYTES (the algorithmic 2 Q s) + 1 kind byte.
 // ---------------------------------------------------------------------------------------------
 template <class L, class S, int CPT>
-__device__ __forceinline__ void bulk_cells(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                           const uint8_t* __restrict__ kind, const KParams& P, uint32_t row, int x0) {
+struct BulkIn {
+  using Raw = typename std::conditional<CPT == 2, typename S::Word, typename S::T>::type;
+  Raw raw[L::Q];
+  uint32_t k;  // kind byte(s) of the lane's cells
+};
+
+// Row coordinates of an item (wave-uniform).
+struct RowPos {
+  uint32_t row;
+  int iy, iz, x0;
+};
+
+__device__ __forceinline__ RowPos item_pos(uint32_t item, const StepArgs& A, const KParams& P, int cpt) {
+  RowPos r;
+  r.row = A.dwpr.div(item);
+  const uint32_t chunk = item - r.row * A.wpr;
+  const uint32_t z = P.divy.div(r.row);
+  r.iz = (int)z;
+  r.iy = (int)(r.row - z * (uint32_t)P.ny);
+  r.x0 = (int)(chunk * 64u + (threadIdx.x & 63u)) * cpt;
+  return r;
+}
+
+template <class L, class S, int CPT>
+__device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin, const uint8_t* __restrict__ kind,
+                                          const KParams& P, const RowPos& r, BulkIn<L, S, CPT>& in) {
   using T = typename S::T;
   constexpr int Q = L::Q;
   constexpr bool D3 = L::DIM == 3;
   const int NX = P.nx, NY = P.ny, NZ = P.nz;
-  if (x0 >= NX) return;
   const int64_t n = P.n;
-  const int iz = (int)(row / (uint32_t)NY);
-  const int iy = (int)row - iz * NY;
-  const int64_t rowoff = (int64_t)row * NX;
-  const int64_t c0 = rowoff + x0;
-  // the kind byte(s) are loaded together with the populations: every source address below is in
-  // bounds whatever the kind, so the loads need not wait for the skip test
-  uint32_t k;
-  if constexpr (CPT == 2) k = *(const uint16_t*)(kind + c0);
-  else k = kind[c0];
-  constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
-  constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
+  const int iy = r.iy, iz = r.iz;
+  // lanes past the row end load the row start instead (in bounds, results unused)
+  const int x0 = r.x0 < NX ? r.x0 : 0;
+  const int64_t c0 = (int64_t)r.row * NX + x0;
+  // every source address below is in bounds whatever the kind, so the loads need not wait for
+  // the skip test
+  if constexpr (CPT == 2) in.k = *(const uint16_t*)(kind + c0);
+  else in.k = kind[c0];
   const bool yLo = iy == 0, yHi = iy == NY - 1;
   const bool zLo = D3 && iz == 0, zHi = D3 && iz == NZ - 1;
-  const bool gwall = yLo && P.ground != 0;
-  // ---- 1. per direction: source direction, row and x shift (wave-uniform, branch-free) ----
+  // per direction: source direction, row and x shift (wave-uniform, branch-free)
   const T* src[Q];
-  int xlim[Q];  // largest x offset a lane may use (outlet clamp for CPT = 1)
+  int xlim[Q];  // largest x a lane may read (outlet clamp for CPT = 1)
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
@@ -232,106 +332,182 @@ __device__ __forceinline__ void bulk_cells(const typename S::T* __restrict__ fin
     src[i] = fin + (int64_t)d * n + ((int64_t)sy + (int64_t)NY * sz) * NX + xs;
     xlim[i] = (grnd || cx >= 0) ? NX - 1 : NX - 2;
   }
-  // ---- 2. all loads in flight together (scalar base + 32-bit lane byte offset) ----
+  // all loads in flight together (scalar base + 32-bit lane byte offset)
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
-  float fa[Q], fb[Q];
   if constexpr (CPT == 2) {
-    typename S::Word raw[Q];
 #pragma unroll
-    for (int i = 0; i < Q; i++) raw[i] = S::ld_word((const T*)((const char*)src[i] + lane_off));
-#pragma unroll
-    for (int i = 0; i < Q; i++) S::unpack_word(raw[i], L::w(i), fa[i], fb[i]);
+    for (int i = 0; i < Q; i++) in.raw[i] = S::ld_word((const T*)((const char*)src[i] + lane_off));
   } else {
-    T raw[Q];
 #pragma unroll
-    for (int i = 0; i < Q; i++) raw[i] = src[i][min(x0, xlim[i])];
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-      fa[i] = S::dev_of(raw[i], L::w(i));
-      fb[i] = 0.0f;
-    }
+    for (int i = 0; i < Q; i++) in.raw[i] = src[i][min(x0, xlim[i])];
   }
+}
+
+template <class L, class S, int CPT, bool REG>
+__device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, const KParams& P, const RowPos& r,
+                                            const BulkIn<L, S, CPT>& in) {
+  using T = typename S::T;
+  constexpr int Q = L::Q;
+  const int NX = P.nx;
+  const int x0 = r.x0;
+  if (x0 >= NX) return;
+  const int64_t n = P.n;
+  const int64_t rowoff = (int64_t)r.row * NX;
+  const uint32_t k = in.k;
+  constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
+  constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
+  const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
+  auto store = [&](int i, float a, float b) {
+    T* o = (T*)((char*)(fout + uniform_i64((int64_t)i * n + rowoff)) + lane_off);
+    if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
+    else S::st_dev(o, 0, L::w(i), a);
+  };
   if (k == SOLID_ALL) {
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
-      T* o = fout + (int64_t)i * n + c0;
-      if constexpr (CPT == 2) S::st2_dev(o, L::w(i), 0.0f, 0.0f);
-      else S::st_dev(o, 0, L::w(i), 0.0f);
-    }
+    for (int i = 0; i < Q; i++) store(i, 0.0f, 0.0f);
     return;
   }
   if (k & SKIP) return;
-  // ---- 3. per-lane fix-ups: outlet clamp, moving belt, inlet ----
+  const bool yLo = r.iy == 0;
+  const bool gwall = yLo && P.ground != 0;
+  const float U = P.uin;
   const float beltA = (P.ground == 2 && !(k & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
   const float beltB = (P.ground == 2 && !((k >> 8) & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
-  const float U = P.uin;
-#pragma unroll
-  for (int i = 1; i < Q; i++) {
+  // Pulled deviation of direction i for cell a (low halves of the loaded words) and cell b (high
+  // halves), with the per-lane fix-ups: moving belt (wgslLbm.ts:159-162), inlet (wgslLbm.ts:146,
+  // checked first there) and, for cell b, the outlet clamp x = nx -> nx - 1, which is the low half
+  // of the same shifted word.
+  auto pull_a = [&](int i) -> float {
+    const int cx = L::c(i, 0), cy = L::c(i, 1);
+    float v;
+    if constexpr (CPT == 2) v = S::lo_dev(in.raw[i], L::w(i));
+    else v = S::dev_of(in.raw[i], L::w(i));
+    if (i > 0) {
+      const bool grnd = cy > 0 && yLo && P.ground != 0;
+      if (grnd && cx != 0) v += beltA * ((6.0f * L::w(i) * cx) * P.uGround);
+      if (cx > 0 && x0 == 0) v = inlet_dev<L>(i, U);
+    }
+    return v;
+  };
+  auto pull_b = [&](int i) -> float {
     const int cx = L::c(i, 0), cy = L::c(i, 1);
     const bool grnd = cy > 0 && yLo && P.ground != 0;
-    if constexpr (CPT == 2) {
-      if (cx < 0 && !grnd && x0 + 2 == NX) fb[i] = fa[i];  // outlet: x = nx clamps to nx - 1
-    }
-    if (grnd && cx != 0) {  // moving belt, wgslLbm.ts:159-162
-      const float m = (6.0f * L::w(i) * cx) * P.uGround;
-      fa[i] += beltA * m;
-      fb[i] += beltB * m;
-    }
-    if (cx > 0 && x0 == 0) fa[i] = inlet_dev<L>(i, U);  // inlet, wgslLbm.ts:146 (checked first there)
-  }
-  collide<L>(fa, (float)x0, gwall, gwall, P);
-  if constexpr (CPT == 2) collide<L>(fb, (float)(x0 + 1), gwall, gwall, P);
+    const bool clamp = i > 0 && cx < 0 && !grnd && x0 + 2 == NX;
+    float v = clamp ? S::lo_dev(in.raw[i], L::w(i)) : S::hi_dev(in.raw[i], L::w(i));
+    if (grnd && cx != 0) v += beltB * ((6.0f * L::w(i) * cx) * P.uGround);
+    return v;
+  };
+  if constexpr (REG) {
+    // Moments of cell a, then of cell b. The regularized outputs need only the moments, so a lane
+    // carries two Relax records (not 2 Q populations) into the output loop, which writes pair by
+    // pair straight to memory.
+    Relax ma, mb;
+    {
+      float f[Q];
 #pragma unroll
-  for (int i = 0; i < Q; i++) {
-    T* o = (T*)((char*)(fout + uniform_i64((int64_t)i * n + rowoff)) + lane_off);
-    if constexpr (CPT == 2) S::st2_dev(o, L::w(i), fa[i], fb[i]);
-    else S::st_dev(o, 0, L::w(i), fa[i]);
+      for (int i = 0; i < Q; i++) f[i] = pull_a(i);
+      ma = relax<L>(f, (float)x0, gwall, true, P);
+    }
+    if constexpr (CPT == 2) {
+      float f[Q];
+#pragma unroll
+      for (int i = 0; i < Q; i++) f[i] = pull_b(i);
+      mb = relax<L>(f, (float)(x0 + 1), gwall, true, P);
+    }
+    store(0, rest_out<L>(ma, true, 0.0f, P), CPT == 2 ? rest_out<L>(mb, true, 0.0f, P) : 0.0f);
+#pragma unroll
+    for (int i = 1; i < Q; i++) {
+      const int o = opp<L>(i);
+      if (o < i) continue;
+      float ai, ao, bi = 0.0f, bo = 0.0f;
+      pair_out<L>(ma, true, i, 0.0f, 0.0f, ai, ao, P);
+      if constexpr (CPT == 2) pair_out<L>(mb, true, i, 0.0f, 0.0f, bi, bo, P);
+      store(i, ai, bi);
+      store(o, ao, bo);
+    }
+  } else {
+    // BGK needs the pre-collision populations: cell a is relaxed in place, then cell b, and the
+    // pairs are stored at the end.
+    float fa[Q], fb[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) fa[i] = pull_a(i);
+    collide_bgk<L>(fa, (float)x0, gwall, P);
+    if constexpr (CPT == 2) {
+#pragma unroll
+      for (int i = 0; i < Q; i++) fb[i] = pull_b(i);
+      collide_bgk<L>(fb, (float)(x0 + 1), gwall, P);
+    }
+#pragma unroll
+    for (int i = 0; i < Q; i++) store(i, fa[i], CPT == 2 ? fb[i] : 0.0f);
   }
 }
 
-// Wall cells: one lane per listed cell (cells next to the body), the reference's full branch
-// logic with precomputed solid masks and link records.
+// Wall cells: one lane per listed cell (cells next to the body). Every pulled population is a
+// precomputed link (two independent gathers and two FMAs), so all loads of a lane are in flight
+// together; then the collision (BGK on wall cells, wgslLbm.ts:186).
 template <class L, class S>
 __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                          const uint8_t* __restrict__ kind, const StepArgs& A, const KParams& P,
-                                          const KWheels& W, int64_t t) {
+                                          const StepArgs& A, const KParams& P, int64_t t) {
+  constexpr int Q = L::Q;
   if (t >= A.nwall) return;
   const int64_t c = A.wlist[t];
-  int ix, iy, iz;
-  coords(c, P, ix, iy, iz);
-  float f[L::Q];
-  update_wall<L, S>(fin, kind[c], A.wmask[t], A.wrec[t], P, W, c, ix, iy, iz, f);
+  const uint32_t meta = A.wmeta[t];
+  const int64_t n = P.n;
+  float f[Q];
+  if (meta & WM_SOLID) {
 #pragma unroll
-  for (int i = 0; i < L::Q; i++) S::st_dev(fout, (int64_t)i * P.n + c, L::w(i), f[i]);
+    for (int i = 0; i < Q; i++) S::st_dev(fout, (int64_t)i * n + c, L::w(i), 0.0f);
+    return;
+  }
+  f[0] = S::ld_dev(fin, c, L::w(0));
+#pragma unroll
+  for (int i = 1; i < Q; i++) {
+    const WallLink k = A.wlink[(int64_t)(i - 1) * A.wcap + t];
+    const float x0 = S::ld_dev(fin, k.e0, L::w(i));
+    const float x1 = S::ld_dev(fin, k.e1, L::w(i));
+    f[i] = fmaf(k.a0, x0, fmaf(k.a1, x1, k.b));
+  }
+  collide<L>(f, (float)(meta >> 8), (meta & WM_WALL) != 0u, (meta & WM_GWALL) != 0u, P);
+#pragma unroll
+  for (int i = 0; i < Q; i++) S::st_dev(fout, (int64_t)i * n + c, L::w(i), f[i]);
 }
 
 // One lattice step in one launch. The first A.nwb blocks update the wall cells (long dependency
-// chains, few lanes: started first so their latency hides under the bulk work); every other block
-// is 4 waves of the bulk update, waves enumerated (row, chunk) with the chunk fastest so a block
-// covers contiguous memory. With A.xcd set, bulk blocks are remapped so that each XCD (blocks are
-// dealt round-robin to the 8 XCDs) walks one contiguous eighth of the grid, keeping the cache
-// lines shared by neighbouring chunks inside one L2.
-template <class L, class S, int CPT>
-__global__ __launch_bounds__(BLOCK) void k_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                                const uint8_t* __restrict__ kind, KParams P, KWheels W, StepArgs A) {
+// chains on few lanes: started first so their latency hides under the bulk work); every other
+// block is 4 waves of the bulk update, one item per wave. Items are numbered (row, chunk) with the
+// chunk fastest. With A.xcd set, bulk block b runs on XCD b % 8 (blocks are dealt round-robin to
+// the 8 XCDs) and that XCD walks one contiguous eighth of the items, its waves side by side, so the
+// cache lines shared by neighbouring rows stay inside one L2.
+// Occupancy target of the step kernel: 6 waves per SIMD (<= 80 VGPRs) for the 16-bit formats,
+// which keeps every path of the kernel free of spills; 4 for FP32 pairs.
+template <class S> constexpr int step_waves() { return S::BYTES == 2 ? 6 : 4; }
+
+template <class L, class S, int CPT, bool REG>
+__global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                const uint8_t* __restrict__ kind, KParams P, StepArgs A) {
   uint32_t b = blockIdx.x;
   if (b < A.nwb) {
-    wall_cell<L, S>(fin, fout, kind, A, P, W, (int64_t)b * BLOCK + threadIdx.x);
+    wall_cell<L, S>(fin, fout, A, P, (int64_t)b * BLOCK + threadIdx.x);
     return;
   }
   b -= A.nwb;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  uint32_t item;
   if (A.xcd) {
-    const uint32_t per = (A.nbb + 7u) / 8u;
-    const uint32_t lb = (b % 8u) * per + b / 8u;
-    if (lb >= A.nbb) return;
-    b = lb;
+    const uint32_t x = b % 8u;
+    const uint32_t j = (b / 8u) * 4u + w;
+    if (j >= A.per) return;
+    item = x * A.per + j;
+  } else {
+    item = b * 4u + w;
   }
-  const uint32_t wave = b * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (wave >= A.nwaves) return;
-  const uint32_t row = wave / A.wpr;
-  const uint32_t chunk = wave - row * A.wpr;
-  const int x0 = (int)(chunk * 64u + (threadIdx.x & 63u)) * CPT;
-  bulk_cells<L, S, CPT>(fin, fout, kind, P, row, x0);
+  if (item >= A.nwaves) return;
+  BulkIn<L, S, CPT> in;
+  const RowPos r = item_pos(item, A, P, CPT);
+  bulk_load<L, S, CPT>(fin, kind, P, r, in);
+  // ground rows relax with BGK (wall cells, wgslLbm.ts:186); the branch is wave-uniform
+  if (REG && !(r.iy == 0 && P.ground != 0)) bulk_finish<L, S, CPT, true>(fout, P, r, in);
+  else bulk_finish<L, S, CPT, false>(fout, P, r, in);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -522,26 +698,17 @@ __global__ __launch_bounds__(BLOCK) void k_pack(const float* __restrict__ in, ty
   f[e] = S::enc(in[e], w);
 }
 
-// Plain streaming copy (16-byte vectors, grid-stride): the HBM-bandwidth calibration kernel.
-// Each lane moves 4 independent 16-byte vectors per iteration so enough bytes are in flight.
+// Plain streaming copy, one 16-byte vector per lane over a grid that covers the buffer: the
+// HBM-bandwidth calibration kernel (the fastest of the simple copy shapes on MI355X).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(BLOCK) void k_copy(const u32x4* __restrict__ a, u32x4* __restrict__ b, int64_t n16) {
-  const int64_t stride = (int64_t)gridDim.x * BLOCK;
-  int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    const u32x4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
-    __builtin_nontemporal_store(v0, &b[i]);
-    __builtin_nontemporal_store(v1, &b[i + stride]);
-    __builtin_nontemporal_store(v2, &b[i + 2 * stride]);
-    __builtin_nontemporal_store(v3, &b[i + 3 * stride]);
-  }
-  for (; i < n16; i += stride) __builtin_nontemporal_store(a[i], &b[i]);
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n16) __builtin_nontemporal_store(a[i], &b[i]);
 }
 
 void launch_copy(hipStream_t st, const void* a, void* b, int64_t bytes) {
   const int64_t n16 = bytes / 16;
-  const unsigned blocks = (unsigned)std::min<int64_t>((n16 + BLOCK - 1) / BLOCK, 256 * 16);
-  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(BLOCK), 0, st, (const u32x4*)a, (u32x4*)b, n16);
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)((n16 + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, (const u32x4*)a, (u32x4*)b, n16);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_clear_stats(float4* __restrict__ meanA, float* __restrict__ meanB, int64_t n) {
@@ -595,40 +762,53 @@ void launch_classify(const LaunchCtx& c, uint32_t* list, uint32_t* masks, unsign
   else go(D3Q19{});
 }
 
-void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, WallRec* recs, int64_t count) {
+void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, WallLink* links, uint32_t* meta, int64_t count,
+                     int64_t cap) {
   if (count <= 0) return;
   if (c.P.q == 9)
-    hipLaunchKernelGGL((k_wallprep<D2Q9>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, c.flags, c.sdf, list, masks, recs, count, c.P);
+    hipLaunchKernelGGL((k_wallprep<D2Q9>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, c.flags, c.sdf, c.kind, list, masks, links,
+                       meta, count, cap, c.P, *c.W);
   else
-    hipLaunchKernelGGL((k_wallprep<D3Q19>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, c.flags, c.sdf, list, masks, recs, count, c.P);
+    hipLaunchKernelGGL((k_wallprep<D3Q19>), dim3(blocks_for(count)), dim3(BLOCK), 0, c.stream, c.flags, c.sdf, c.kind, list, masks, links,
+                       meta, count, cap, c.P, *c.W);
 }
 
-StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, const WallRec* recs, int64_t count, int xcd) {
+StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
+                   int xcd) {
   StepArgs A;
   const int cpt = cells_per_thread(c.P);
   A.wlist = list;
-  A.wmask = masks;
-  A.wrec = recs;
+  A.wmeta = meta;
+  A.wlink = links;
+  A.wcap = cap;
   A.nwall = count;
   A.nwb = (uint32_t)(((count + BLOCK - 1) / BLOCK + 7) / 8 * 8);  // a multiple of 8 keeps bulk blocks XCD-aligned
   A.wpr = (uint32_t)((c.P.nx + 64 * cpt - 1) / (64 * cpt));
+  A.dwpr.init(A.wpr);
   A.nwaves = (uint32_t)((int64_t)c.P.ny * c.P.nz * A.wpr);
-  A.nbb = (A.nwaves + 3u) / 4u;
+  A.per = (A.nwaves + 7u) / 8u;
+  // with the XCD placement every XCD gets ceil(per / 4) blocks
+  A.nbb = xcd ? 8u * ((A.per + 3u) / 4u) : (A.nwaves + 3u) / 4u;
   A.xcd = xcd;
   return A;
 }
 
 void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A) {
   const int cpt = cells_per_thread(c.P);
-  const dim3 grid(A.nwb + (A.xcd ? (A.nbb + 7u) / 8u * 8u : A.nbb));
+  const bool reg = c.P.collision != 1;
+  const dim3 grid(A.nwb + A.nbb);
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
-    if (cpt == 2)
-      hipLaunchKernelGGL((k_step<L, S, 2>), grid, dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind,
-                         c.P, *c.W, A);
-    else
-      hipLaunchKernelGGL((k_step<L, S, 1>), grid, dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind,
-                         c.P, *c.W, A);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind, c.P, A);
+    };
+    if (cpt == 2) {
+      if (reg) go(k_step<L, S, 2, true>);
+      else go(k_step<L, S, 2, false>);
+    } else {
+      if (reg) go(k_step<L, S, 1, true>);
+      else go(k_step<L, S, 1, false>);
+    }
   });
 }
 

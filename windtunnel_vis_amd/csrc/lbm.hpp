@@ -112,6 +112,8 @@ struct SF32 {
     a = __uint_as_float((uint32_t)v) - w;
     b = __uint_as_float((uint32_t)(v >> 32)) - w;
   }
+  __device__ static float lo_dev(Word v, float w) { return __uint_as_float((uint32_t)v) - w; }
+  __device__ static float hi_dev(Word v, float w) { return __uint_as_float((uint32_t)(v >> 32)) - w; }
   __device__ static float dev_of(T v, float w) { return v - w; }
 };
 
@@ -132,6 +134,8 @@ struct SF16 {
   using Word = uint32_t;
   __device__ static Word ld_word(const T* p) { return *(const u32_a2*)p; }
   __device__ static void unpack_word(Word v, float, float& a, float& b) { unpack_h2(v, a, b); }
+  __device__ static float lo_dev(Word v, float) { return (float)__builtin_bit_cast(f16x2, v).x; }
+  __device__ static float hi_dev(Word v, float) { return (float)__builtin_bit_cast(f16x2, v).y; }
   __device__ static float dev_of(T v, float) { return h2f(v); }
 };
 
@@ -152,6 +156,8 @@ struct SBF16 {
   using Word = uint32_t;
   __device__ static Word ld_word(const T* p) { return *(const u32_a2*)p; }
   __device__ static void unpack_word(Word v, float, float& a, float& b) { unpack_b2(v, a, b); }
+  __device__ static float lo_dev(Word v, float) { return __uint_as_float(v << 16); }
+  __device__ static float hi_dev(Word v, float) { return __uint_as_float(v & 0xffff0000u); }
   __device__ static float dev_of(T v, float) { return b2f(v); }
 };
 
@@ -195,24 +201,32 @@ struct KWheels {
   float4 w[32];  // w[2k] = (centre, 1/R), w[2k+1] = (axis, half width), SolverGPU.ts:325-335
 };
 
-// Precomputed per-link data of a cell next to the body (see wall_record in kernels_lbm.hip).
-struct WallRec {
-  float q[19];       // Bouzidi fraction per direction (1/2 for halfway links and non-solid links)
-  uint8_t tag[20];   // wheel index + 1 of the solid cell a link points into (0: not a wheel)
+// One pulled population of a cell next to the body, precomputed from the reference's branch logic
+// (inlet, ground + belt, mirrors, periodic z, halfway / Bouzidi walls, rotating wheels with the
+// per-cell mass fix; see k_wallprep): d_i = a0 d[e0] + a1 d[e1] + b, with e0 / e1 element indices
+// (direction * n + cell) into the population array. A link with one source has e1 = e0, a1 = 0.
+struct alignas(16) WallLink {
+  int64_t e0, e1;
+  float a0, a1, b;
+  uint32_t pad;
 };
+// per wall cell: meta = WM_* bits | x << 8
+enum : uint32_t { WM_SOLID = 1u, WM_WALL = 2u, WM_GWALL = 4u };
 
 // Launch layout of one step (see k_step): wall blocks first, then bulk blocks of 4 waves.
 struct StepArgs {
-  const uint32_t* wlist;
-  const uint32_t* wmask;
-  const WallRec* wrec;
+  const uint32_t* wlist;   // wall cells
+  const uint32_t* wmeta;   // their meta words
+  const WallLink* wlink;   // their links, direction-major: wlink[(i - 1) * wcap + t]
   int64_t nwall;
-  uint32_t nwb;     // wall blocks (multiple of 8)
+  int64_t wcap;
+  uint32_t nwb;     // wall blocks
   uint32_t wpr;     // waves per x-row
-  uint32_t nwaves;  // bulk waves
+  uint32_t nwaves;  // bulk items (one wave-sized chunk of an x-row each)
   uint32_t nbb;     // bulk blocks
-  int32_t xcd;      // XCD-aware remap of bulk blocks
-  int32_t pad;
+  uint32_t per;     // items per XCD, ceil(nwaves / 8)
+  int32_t xcd;      // XCD-aware placement of bulk items
+  FastDiv dwpr;     // division by wpr
 };
 
 // wgslLbm.ts:21-28
@@ -270,10 +284,24 @@ __device__ __forceinline__ float csum(int a, float x, int b, float y, int c, flo
   return r;
 }
 
+// Moments and relaxation parameters of one cell: everything the post-collision populations need
+// besides (for BGK) the pre-collision ones. `reg` selects the regularized form.
+struct Relax {
+  float E;             // even part shared by all directions (mode-dependent, see relax)
+  float rho45;         // 4.5 rho
+  float ux, uy, uz;    // velocity
+  float jx, jy, jz;    // momentum (deviation form: j = rho u)
+  float pxx, pyy, pzz, pxy, pxz, pyz;  // Pi^neq
+  float k1;            // 4.5 (1 - omega) (regularized)
+  float omega, om1;    // 1 / tau, 1 - 1 / tau
+  float sIn;           // inlet absorbing-layer weight (0 outside the layer)
+};
+
 template <class L>
-__device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall, bool gwall, const KParams& P) {
+__device__ __forceinline__ Relax relax(const float (&d)[L::Q], float xpos, bool gwall, bool reg, const KParams& P) {
   constexpr int Q = L::Q;
   constexpr bool D3 = L::DIM == 3;
+  Relax m;
   // pair sums / differences
   float s[Q], a[Q];
   float drho = d[0];
@@ -328,144 +356,100 @@ __device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall,
   const float omega = fast_rcp(tau);
   const float om1 = 1.0f - omega;
   const float si = __builtin_amdgcn_fmed3f(1.0f - xpos * P.invSpongeIn, 0.0f, 1.0f);
-  const float sIn = P.spongeIn > 0.0f ? 0.08f * si * si : 0.0f;
-  const float rho45 = 4.5f * rho;
-  // (p_aa + p_bb) shared by the diagonal directions of each plane
-  const float pxy_d = pxx + pyy, pxz_d = pxx + pzz, pyz_d = pyy + pzz;
-  float out[Q];
-  if (!(wall || P.collision == 1)) {
-    // regularized: f_i - w_i = w_i (E + 4.5 rho cu^2 + k1 Q_i:Pi) +- 3 w_i c_i.j
-    const float k1 = 4.5f * om1;
+  m.sIn = P.spongeIn > 0.0f ? 0.08f * si * si : 0.0f;
+  m.rho45 = 4.5f * rho;
+  m.ux = ux; m.uy = uy; m.uz = uz;
+  m.jx = jx; m.jy = jy; m.jz = jz;
+  m.pxx = pxx; m.pyy = pyy; m.pzz = pzz; m.pxy = pxy; m.pxz = pxz; m.pyz = pyz;
+  m.omega = omega;
+  m.om1 = om1;
+  if (reg) {
+    m.k1 = 4.5f * om1;
     const float trP = D3 ? (pxx + pyy + pzz) * third : (pxx + pyy) * third;
-    const float E = drho - 1.5f * rho * usq - k1 * trP;
-    out[0] = L::w(0) * E;
-#pragma unroll
-    for (int i = 1; i < Q; i++) {
-      const int o = opp<L>(i);
-      if (o < i) continue;
-      const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-      const float cu = csum(cx, ux, cy, uy, cz, uz);
-      const float cj = csum(cx, jx, cy, jy, cz, jz);
-      // Q_i : Pi for |c| = 1 (one diagonal entry) or |c|^2 = 2 (two diagonal entries + 2 off-diagonal)
-      float qp;
-      if (cx && cy) qp = pxy_d + (float)(2 * cx * cy) * pxy;
-      else if (cx && cz) qp = pxz_d + (float)(2 * cx * cz) * pxz;
-      else if (cy && cz) qp = pyz_d + (float)(2 * cy * cz) * pyz;
-      else qp = cx ? pxx : (cy ? pyy : pzz);
-      const float ev = L::w(i) * (E + rho45 * cu * cu + k1 * qp);
-      const float od = (3.0f * L::w(i)) * cj;
-      out[i] = ev + od;
-      out[o] = ev - od;
-    }
+    m.E = drho - 1.5f * rho * usq - m.k1 * trP;
   } else {
-    // BGK: f_i - w_i = omega Deq_i + (1 - omega) d_i
-    const float E = drho - 1.5f * rho * usq;
-    out[0] = omega * (L::w(0) * E) + om1 * d[0];
-#pragma unroll
-    for (int i = 1; i < Q; i++) {
-      const int o = opp<L>(i);
-      if (o < i) continue;
-      const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-      const float cu = csum(cx, ux, cy, uy, cz, uz);
-      const float cj = csum(cx, jx, cy, jy, cz, jz);
-      const float ev = L::w(i) * (E + rho45 * cu * cu);
-      const float od = (3.0f * L::w(i)) * cj;
-      out[i] = (ev + od) + om1 * (d[i] - (ev + od));
-      out[o] = (ev - od) + om1 * (d[o] - (ev - od));
-    }
+    m.k1 = 0.0f;
+    m.E = drho - 1.5f * rho * usq;
   }
-  if (sIn > 0.0f) {  // inlet absorbing layer: mix toward feq(1, U), wgslLbm.ts:220, :238
-    const float U = P.uin;
+  return m;
+}
+
+// Inlet absorbing layer: mix toward feq(1, U), wgslLbm.ts:220, :238.
+template <class L>
+__device__ __forceinline__ float inlet_mix(const Relax& m, int i, float out, const KParams& P) {
+  return m.sIn > 0.0f ? out + m.sIn * (inlet_dev<L>(i, P.uin) - out) : out;
+}
+
+// Post-collision deviation of the rest direction.
+template <class L>
+__device__ __forceinline__ float rest_out(const Relax& m, bool reg, float d0, const KParams& P) {
+  const float o = reg ? L::w(0) * m.E : m.omega * (L::w(0) * m.E) + m.om1 * d0;
+  return inlet_mix<L>(m, 0, o, P);
+}
+
+// Post-collision deviations of the pair (i, opp i), i < opp i:
+// regularized: w_i (E + 4.5 rho cu^2 + k1 Q_i:Pi) +- 3 w_i c_i.j; BGK: omega Deq + (1 - omega) d.
+template <class L>
+__device__ __forceinline__ void pair_out(const Relax& m, bool reg, int i, float di, float dop, float& oi, float& oo,
+                                         const KParams& P) {
+  const int o = opp<L>(i);
+  const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
+  const float cu = csum(cx, m.ux, cy, m.uy, cz, m.uz);
+  const float cj = csum(cx, m.jx, cy, m.jy, cz, m.jz);
+  const float od = (3.0f * L::w(i)) * cj;
+  if (reg) {
+    // Q_i : Pi for |c| = 1 (one diagonal entry) or |c|^2 = 2 (two diagonal entries + 2 off-diagonal)
+    float qp;
+    if (cx && cy) qp = (m.pxx + m.pyy) + (float)(2 * cx * cy) * m.pxy;
+    else if (cx && cz) qp = (m.pxx + m.pzz) + (float)(2 * cx * cz) * m.pxz;
+    else if (cy && cz) qp = (m.pyy + m.pzz) + (float)(2 * cy * cz) * m.pyz;
+    else qp = cx ? m.pxx : (cy ? m.pyy : m.pzz);
+    const float ev = L::w(i) * (m.E + m.rho45 * cu * cu + m.k1 * qp);
+    oi = ev + od;
+    oo = ev - od;
+  } else {
+    const float ev = L::w(i) * (m.E + m.rho45 * cu * cu);
+    oi = (ev + od) + m.om1 * (di - (ev + od));
+    oo = (ev - od) + m.om1 * (dop - (ev - od));
+  }
+  oi = inlet_mix<L>(m, i, oi, P);
+  oo = inlet_mix<L>(m, o, oo, P);
+}
+
+// BGK collision on deviations in place: f_i - w_i = omega Deq_i + (1 - omega) d_i.
+template <class L>
+__device__ __forceinline__ void collide_bgk(float (&d)[L::Q], float xpos, bool gwall, const KParams& P) {
+  constexpr int Q = L::Q;
+  const Relax m = relax<L>(d, xpos, gwall, false, P);
+  d[0] = rest_out<L>(m, false, d[0], P);
 #pragma unroll
-    for (int i = 0; i < Q; i++) out[i] = out[i] + sIn * (inlet_dev<L>(i, U) - out[i]);
+  for (int i = 1; i < Q; i++) {
+    const int o = opp<L>(i);
+    if (o < i) continue;
+    float oi, oo;
+    pair_out<L>(m, false, i, d[i], d[o], oi, oo, P);
+    d[i] = oi;
+    d[o] = oo;
+  }
+}
+
+// Collision on deviations in place (wall cells use BGK, wgslLbm.ts:186).
+template <class L>
+__device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall, bool gwall, const KParams& P) {
+  constexpr int Q = L::Q;
+  const bool reg = !(wall || P.collision == 1);
+  const Relax m = relax<L>(d, xpos, gwall, reg, P);
+  float out[Q];
+  out[0] = rest_out<L>(m, reg, d[0], P);
+#pragma unroll
+  for (int i = 1; i < Q; i++) {
+    const int o = opp<L>(i);
+    if (o < i) continue;
+    pair_out<L>(m, reg, i, d[i], d[o], out[i], out[o], P);
   }
 #pragma unroll
   for (int i = 0; i < Q; i++) d[i] = out[i];
 }
 
-// ---------------------------------------------------------------------------------------------
-// General per-cell pull + boundaries (wgslLbm.ts:139-175, :262-291), the reference's branch order,
-// for cells next to the body. `smask` bit i is set when the plain pull of direction i comes from a
-// solid cell (precomputed from flags). Produces post-collision deviations.
-// ---------------------------------------------------------------------------------------------
-template <class L, class S>
-__device__ inline void update_wall(const typename S::T* __restrict__ fin, uint8_t kd, uint32_t smask, const WallRec& R,
-                                   const KParams& P, const KWheels& W, int64_t idx, int ix, int iy, int iz, float (&f)[L::Q]) {
-  constexpr int Q = L::Q, DIM = L::DIM;
-  const int64_t n = P.n;
-  const int NX = P.nx, NY = P.ny, NZ = P.nz;
-  if (kd & KIND_SOLID) {
-#pragma unroll
-    for (int i = 0; i < Q; i++) f[i] = 0.0f;
-    return;
-  }
-  auto cell = [&](int x, int y, int z) -> int64_t { return (int64_t)x + (int64_t)NX * ((int64_t)y + (int64_t)NY * z); };
-  const float U = P.uin;
-  bool wall = false, gwall = false;
-  float mvSum = 0.f, mvW = 0.f;
-  uint32_t wm = 0u;
-  const float belt = (P.ground == 2 && !(kd & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
-  f[0] = S::ld_dev(fin, idx, L::w(0));
-#pragma unroll
-  for (int i = 1; i < Q; i++) {
-    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-    const float w = L::w(i);
-    int sx = ix - cx, sy = iy - cy, sz = DIM == 3 ? iz - cz : 0;
-    if (cx < 0) sx = min(sx, NX - 1);
-    const bool yOut = cy > 0 ? sy < 0 : (cy < 0 ? sy >= NY : false);
-    const bool zOut = DIM == 3 ? (cz > 0 ? sz < 0 : (cz < 0 ? sz >= NZ : false)) : false;
-    float v;
-    if (cx > 0 && sx < 0) {
-      v = inlet_dev<L>(i, U);
-    } else if (cy > 0 && yOut && P.ground != 0) {
-      wall = true; gwall = true;
-      v = S::ld_dev(fin, (int64_t)opp<L>(i) * n + idx, w);
-      if (cx != 0) v += belt * ((6.0f * w * cx) * P.uGround);
-    } else {
-      const int sxc = min(max(sx, 0), NX - 1);
-      if (DIM == 3 && yOut && zOut) {
-        v = S::ld_dev(fin, (int64_t)mirYZ<L>(i) * n + cell(sxc, iy, iz), w);
-      } else if (yOut) {
-        v = S::ld_dev(fin, (int64_t)mirY<L>(i) * n + cell(sxc, iy, DIM == 3 ? min(max(sz, 0), NZ - 1) : 0), w);
-      } else if (DIM == 3 && zOut) {
-        if (P.sideMode == 1) v = S::ld_dev(fin, (int64_t)i * n + cell(sxc, sy, (sz + NZ) % NZ), w);
-        else v = S::ld_dev(fin, (int64_t)mirZ<L>(i) * n + cell(sxc, sy, iz), w);
-      } else {
-        const int64_t sc = cell(sxc, sy, sz);
-        if (smask & (1u << i)) {
-          wall = true;
-          // Bouzidi (q < 1/2 with the second fluid node x + c_i, q > 1/2 with f_i(x)); the record
-          // already holds q = 1/2 where the reference falls back to halfway bounce-back
-          const float fb = S::ld_dev(fin, (int64_t)opp<L>(i) * n + idx, w);
-          const float q = P.interp ? R.q[i] : 0.5f;
-          v = fb;
-          if (q < 0.5f) {
-            const int ax = ix + cx, ay = iy + cy, az = DIM == 3 ? iz + cz : 0;
-            v = 2.0f * q * fb + (1.0f - 2.0f * q) * S::ld_dev(fin, (int64_t)opp<L>(i) * n + cell(ax, ay, az), w);
-          } else if (q > 0.5f) {
-            v = (0.5f / q) * fb + (1.0f - 0.5f / q) * S::ld_dev(fin, (int64_t)i * n + idx, w);
-          }
-          if (R.tag[i] != 0) {
-            const float3 p = make_float3(0.5f * (float)(ix + sxc) + 0.5f, 0.5f * (float)(iy + sy) + 0.5f,
-                                         0.5f * (float)(iz + sz) + 0.5f);
-            const float3 uw = wheel_vel(W, P, (uint32_t)R.tag[i] - 1u, p);
-            const float mv = (6.0f * w) * ((float)cx * uw.x + (float)cy * uw.y + (float)cz * uw.z) * (q > 0.5f ? 0.5f / q : 1.0f);
-            v += mv; mvSum += mv; mvW += w; wm |= 1u << i;
-          }
-        } else {
-          v = S::ld_dev(fin, (int64_t)i * n + sc, w);
-        }
-      }
-    }
-    f[i] = v;
-  }
-  if (mvW > 0.0f) {  // wheel mass fix, wgslLbm.ts:289-291
-    const float corr = mvSum / mvW;
-#pragma unroll
-    for (int i = 1; i < Q; i++)
-      if (wm & (1u << i)) f[i] -= L::w(i) * corr;
-  }
-  collide<L>(f, (float)ix, wall, gwall, P);
-}
 
 }  // namespace wt

@@ -48,8 +48,9 @@ struct wt_solver {
   uint8_t* kind = nullptr;
   uint32_t* wlist = nullptr;   // cells handled by the wall kernel
   uint32_t* wmask = nullptr;   // their solid masks
-  WallRec* wrec = nullptr;     // their per-link records
-  int64_t wrec_cap = 0;
+  WallLink* wlink = nullptr;   // their links, (q - 1) x wcap
+  uint32_t* wmeta = nullptr;   // their meta words
+  int64_t wcap = 0;
   unsigned long long* counts = nullptr;
   ushort4* vel = nullptr;
   float4* meanA = nullptr;
@@ -65,7 +66,7 @@ struct wt_solver {
   // profiling
   bool prof = false;
   std::vector<hipEvent_t> ev;   // per step: start, stop of the step kernel
-  int xcd_remap = 1;
+  int xcd_remap = 1;            // WT_TUNE_XCD_REMAP
   size_t ev_used = 0;
   double acc_bulk = 0, acc_bnd = 0;
   int64_t acc_launches = 0;
@@ -137,10 +138,11 @@ int run_init(wt_solver* s) {
   return WT_OK;
 }
 
-// Per-link records of the wall cells (depend on flags and distances).
+// Links of the wall cells (depend on flags, distances, wheels and parameters).
 int prep_walls(wt_solver* s) {
+  if (!s->wlink) return WT_OK;
   LaunchCtx c = s->ctx();
-  launch_wallprep(c, s->wlist, s->wmask, s->wrec, s->nwall);
+  launch_wallprep(c, s->wlist, s->wmask, s->wlink, s->wmeta, s->nwall, s->wcap);
   HIP_TRY(hipGetLastError());
   return WT_OK;
 }
@@ -156,13 +158,17 @@ int reclassify(wt_solver* s) {
   s->nwall = (int64_t)h[0];
   s->nfast = (int64_t)h[1];
   s->nsolid = (int64_t)h[2];
-  if (s->nwall > s->wrec_cap) {
-    if (s->wrec) HIP_TRY(hipFree(s->wrec));
-    s->wrec = nullptr;
-    s->wrec_cap = 0;
+  if (s->nwall > s->wcap || !s->wlink) {
+    if (s->wlink) HIP_TRY(hipFree(s->wlink));
+    if (s->wmeta) HIP_TRY(hipFree(s->wmeta));
+    s->wlink = nullptr;
+    s->wmeta = nullptr;
+    s->wcap = 0;
     const int64_t cap = s->nwall + s->nwall / 4 + 1024;
-    if (hipMalloc((void**)&s->wrec, sizeof(WallRec) * (size_t)cap) != hipSuccess) return fail(WT_E_NOMEM, "wall record allocation failed");
-    s->wrec_cap = cap;
+    if (hipMalloc((void**)&s->wlink, sizeof(WallLink) * (size_t)(s->q - 1) * (size_t)cap) != hipSuccess ||
+        hipMalloc((void**)&s->wmeta, sizeof(uint32_t) * (size_t)cap) != hipSuccess)
+      return fail(WT_E_NOMEM, "wall link allocation failed");
+    s->wcap = cap;
   }
   return prep_walls(s);
 }
@@ -173,13 +179,13 @@ int do_steps(wt_solver* s, int steps) {
     if (r) return r;
   }
   LaunchCtx c = s->ctx();
-  const StepArgs A = step_args(c, s->wlist, s->wmask, s->wrec, s->nwall, s->xcd_remap);
+  const StepArgs A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->xcd_remap);
   for (int k = 0; k < steps; k++) {
     const void* fin = s->f[s->parity];
     void* fout = s->f[s->parity ^ 1];
     if (s->prof) {
       if (s->ev_used + 2 > s->ev.size()) {
-        size_t old = s->ev.size();
+        const size_t old = s->ev.size();
         s->ev.resize(std::max<size_t>(64, old * 2));
         for (size_t i = old; i < s->ev.size(); i++) HIP_TRY(hipEventCreate(&s->ev[i]));
       }
@@ -282,13 +288,13 @@ int wt_create(const wt_dims* dims, const wt_params* params, wt_solver** out) {
 
 int wt_destroy(wt_solver* s) {
   if (!s) return WT_OK;
-  if (s->stream) hipStreamSynchronize(s->stream);
-  for (auto e : s->ev) hipEventDestroy(e);
-  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wrec, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto e : s->ev) (void)hipEventDestroy(e);
+  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
   for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (s->h_result) hipHostFree(s->h_result);
-  if (s->stream) hipStreamDestroy(s->stream);
+    if (p) (void)hipFree(p);
+  if (s->h_result) (void)hipHostFree(s->h_result);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return WT_OK;
 }
@@ -325,7 +331,7 @@ int wt_set_params(wt_solver* s, const wt_params* p, uint32_t mask) {
   np.precision = s->prec;  // fixed at construction
   s->prm = np;
   derive(s);
-  return WT_OK;
+  return prep_walls(s);  // the wall links hold parameter-dependent terms
 }
 
 int wt_get_params(const wt_solver* s, wt_params* out) {
@@ -405,7 +411,7 @@ int wt_set_wheels(wt_solver* s, const float* w, int32_t count) {
     s->W.w[2 * k] = make_float4(e[0], e[1], e[2], 1.0f / std::max(e[6], 1e-3f));
     s->W.w[2 * k + 1] = make_float4(e[3], e[4], e[5], e[7]);
   }
-  return WT_OK;
+  return prep_walls(s);  // the wall links hold the wheel velocities
 }
 
 int wt_step(wt_solver* s, int32_t steps) {
@@ -538,8 +544,8 @@ int wt_copy_probe(int64_t bytes, int32_t iters, double* gbs) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
-    if (a) hipFree(a);
-    hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    (void)hipStreamDestroy(st);
     return fail(WT_E_NOMEM, "copy probe allocation failed");
   }
   HIP_TRY(hipMemsetAsync(a, 1, bytes, st));
@@ -553,23 +559,25 @@ int wt_copy_probe(int64_t bytes, int32_t iters, double* gbs) {
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
   *gbs = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(b);
-  hipStreamDestroy(st);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipStreamDestroy(st);
   return WT_OK;
 }
 
 int wt_set_tuning(wt_solver* s, int32_t key, int32_t value) {
   if (!s) return fail(WT_E_ARG, "null argument");
   if (key == WT_TUNE_XCD_REMAP) { s->xcd_remap = value ? 1 : 0; return WT_OK; }
+
   return fail(WT_E_ARG, "unknown tuning key");
 }
 
 int wt_get_tuning(const wt_solver* s, int32_t key, int32_t* value) {
   if (!s || !value) return fail(WT_E_ARG, "null argument");
   if (key == WT_TUNE_XCD_REMAP) { *value = s->xcd_remap; return WT_OK; }
+
   return fail(WT_E_ARG, "unknown tuning key");
 }
 
