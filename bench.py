@@ -54,9 +54,16 @@ def traffic_from_profiles(name):
     return None
 
 
-def run_gpu(dims, steps, warmup, precision="f16", xcd=1):
+def run_gpu(dims, steps, warmup, precision="f16", chunk=None):
+    """Time `steps` steps. chunk None: launch configuration from the autotuner; else that item-run
+    length (4 = plain order, 0 = one contiguous eighth per XCD)."""
+    from windtunnel_vis_amd import tuning
     s, body, info = make_sedan_solver(dims, precision)
-    s.setTuning(0, xcd)
+    tune = None
+    if chunk is None:
+        tune = tuning.autotune(s, steps=20)
+    else:
+        s.setTuning(0, chunk)
     q = s.lattice.q
     sbytes = 2 if s.precision != "f32" else 4
     n = s.n
@@ -80,7 +87,7 @@ def run_gpu(dims, steps, warmup, precision="f16", xcd=1):
     achieved = k_bytes / k_avg_s / 1e9
     return dict(solver=s, body=body, info=info, mlups=mlups, wall=wall, k_avg_ms=k_avg_s * 1e3, fluid=fluid,
                 bulk_cells=bulk_cells, wall_cells=wall_cells, k_bytes=k_bytes, achieved_gbs=achieved,
-                unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes)
+                unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes, tune=tune)
 
 
 def copy_probe(nbytes=2 << 30, iters=5):
@@ -153,11 +160,7 @@ def main():
         args.no_physics = args.no_cpu = True
         args.north_star = False
     copy_gbs = copy_probe(256 << 20, 3) if args.profile_run else copy_probe()
-    r = run_gpu(dims, args.steps, args.warmup, args.precision, xcd=1)
-    rx = None
-    if not args.profile_run:
-        rx = run_gpu(dims, max(args.steps // 4, 200), args.warmup, args.precision, xcd=0)
-        rx["solver"].destroy()
+    r = run_gpu(dims, args.steps, args.warmup, args.precision)
     out = {
         "metric": METRIC, "value": round(r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 5), "higher_is_better": True,
@@ -175,8 +178,11 @@ def main():
                        "measured_copy_gbs": round(copy_gbs, 1)}
     out["step_roofline_frac"] = round(r["mlups"] * 1e6 * 2 * r["q"] * r["sbytes"] / (HBM_PEAK_GBS * 1e9), 4)
     out["work_split"] = {"bulk_fluid_cells": r["bulk_cells"], "wall_cells": r["wall_cells"]}
-    if rx:
-        out["xcd_remap_off_mlups"] = round(rx["mlups"], 1)
+    if r["tune"]:
+        t = r["tune"]
+        out["tuning"] = {"chunk": t["chunk"], "cells_per_thread": t["cellsPerThread"], "cached": t["cached"],
+                         "trials_ms_per_step": {"%d/%d" % (e["cellsPerThread"], e["chunk"]): round(e["msPerStep"], 5)
+                                                for e in t["trials"]}}
     out["unstable"] = bool(r["unstable"])
     if not args.no_physics:
         ph = physics(r["solver"], r["body"], r["info"], 0.08)
@@ -186,7 +192,7 @@ def main():
     if args.north_star:
         big = (512, 256, 256)
         rb = run_gpu(big, max(args.steps // 4, 200), args.warmup, args.precision)
-        out["north_star_512x256x256"] = {"mlups": round(rb["mlups"], 1),
+        out["north_star_512x256x256"] = {"mlups": round(rb["mlups"], 1), "chunk": rb["tune"]["chunk"],
                                           "step_roofline_frac": round(rb["mlups"] * 1e6 * 2 * rb["q"] * rb["sbytes"] / (HBM_PEAK_GBS * 1e9), 4),
                                           "kernel_frac": round(rb["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                           "avg_launch_ms": round(rb["k_avg_ms"], 5)}

@@ -224,10 +224,30 @@ int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cel
 int wt_copy_probe(int64_t bytes, int32_t iters, double* gbs);
 
 /* Launch tuning knobs (no reference counterpart; set by the autotuner).
- * WT_TUNE_XCD_REMAP: 1 places each eighth of the bulk items on one XCD (default 1). */
-enum { WT_TUNE_XCD_REMAP = 0 };
+ * WT_TUNE_ITEM_CHUNK: the bulk items (64 or 128-cell chunks of x-rows) are cut into runs of this
+ * many consecutive items, dealt round-robin to the 8 XCDs; 4 is the plain order, 0 (default) gives
+ * every XCD one contiguous eighth of the grid.
+ * WT_TUNE_CELLS_PER_THREAD: 2 (default; pairs along x, needs an even nx) or 1. FP16 / BF16
+ * storage always uses 2. Changing it rebuilds the wall-cell list. */
+enum { WT_TUNE_ITEM_CHUNK = 0, WT_TUNE_CELLS_PER_THREAD = 1 };
 int wt_set_tuning(wt_solver* s, int32_t key, int32_t value);
 int wt_get_tuning(const wt_solver* s, int32_t key, int32_t* value);
+
+/* Autotuner (no reference counterpart): times `steps` steps for every candidate launch
+ * configuration (cells per thread x item-run length), keeps the fastest on the handle and reports
+ * all trials. The populations, step counter and profiling state are restored afterwards. */
+#define WT_TUNE_MAX_TRIALS 16
+typedef struct wt_tune_result {
+  int32_t chunk;             /* chosen WT_TUNE_ITEM_CHUNK */
+  int32_t cells_per_thread;  /* chosen WT_TUNE_CELLS_PER_THREAD */
+  double ms_per_step;        /* its time per step */
+  int32_t trials;
+  int32_t pad;
+  int32_t trial_chunk[WT_TUNE_MAX_TRIALS];
+  int32_t trial_cpt[WT_TUNE_MAX_TRIALS];
+  double trial_ms[WT_TUNE_MAX_TRIALS];
+} wt_tune_result;
+int wt_autotune(wt_solver* s, int32_t steps, wt_tune_result* out);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,10 @@ PRECISION_NAME = {v: k for k, v in PRECISION.items()}
 COLLISION = {"regularized": 0, "bgk": 1}
 CLOSURE = {"smagorinsky": 0, "mlp": 1}
 
+TUNE_ITEM_CHUNK = 0
+TUNE_CELLS_PER_THREAD = 1
+TUNE_MAX_TRIALS = 16
+
 PRIM_ROUNDED_BOX = 1
 PRIM_CYLINDER = 2
 PRIM_SPHERE = 3
@@ -60,6 +64,12 @@ class Info(C.Structure):
 
 class VoxelInfo(C.Structure):
     _fields_ = [("frontal", C.c_int64), ("solid_cells", C.c_int64), ("min", C.c_int32 * 3), ("max", C.c_int32 * 3)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("chunk", C.c_int32), ("cells_per_thread", C.c_int32), ("ms_per_step", C.c_double),
+                ("trials", C.c_int32), ("pad", C.c_int32), ("trial_chunk", C.c_int32 * 16),
+                ("trial_cpt", C.c_int32 * 16), ("trial_ms", C.c_double * 16)]
 
 
 class Prim(C.Structure):
@@ -102,6 +112,7 @@ SIGNATURES = {
     "wt_copy_probe": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "wt_set_tuning": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "wt_get_tuning": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    "wt_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(TuneResult)]),
 }
 
 _lib = None

@@ -472,14 +472,13 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
   for (int i = 0; i < Q; i++) S::st_dev(fout, (int64_t)i * n + c, L::w(i), f[i]);
 }
 
-// One lattice step in one launch. The first A.nwb blocks update the wall cells (long dependency
-// chains on few lanes: started first so their latency hides under the bulk work); every other
-// block is 4 waves of the bulk update, one item per wave. Items are numbered (row, chunk) with the
-// chunk fastest. With A.xcd set, bulk block b runs on XCD b % 8 (blocks are dealt round-robin to
-// the 8 XCDs) and that XCD walks one contiguous eighth of the items, its waves side by side, so the
-// cache lines shared by neighbouring rows stay inside one L2.
-// Occupancy target of the step kernel: 6 waves per SIMD (<= 80 VGPRs) for the 16-bit formats,
-// which keeps every path of the kernel free of spills; 4 for FP32 pairs.
+// One lattice step in one launch. The first A.nwb blocks update the wall cells (short gathers on
+// few lanes: started first so their latency hides under the bulk work); every other block is 4
+// waves of the bulk update, one item per wave. Items are numbered (row, chunk) with the chunk
+// fastest. Bulk block b runs on XCD b % 8 (blocks are dealt round-robin to the 8 XCDs); the items
+// are cut into runs of A.xk consecutive items, the runs dealt round-robin to the XCDs, so that
+// the cache lines shared by neighbouring rows of one run stay inside one L2. A.xk = 4 is the plain
+// order (block b takes items 4b..4b+3); A.xk = nwaves / 8 gives every XCD one contiguous eighth.
 template <class S> constexpr int step_waves() { return S::BYTES == 2 ? 6 : 4; }
 
 template <class L, class S, int CPT, bool REG>
@@ -492,15 +491,9 @@ __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename 
   }
   b -= A.nwb;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  uint32_t item;
-  if (A.xcd) {
-    const uint32_t x = b % 8u;
-    const uint32_t j = (b / 8u) * 4u + w;
-    if (j >= A.per) return;
-    item = x * A.per + j;
-  } else {
-    item = b * 4u + w;
-  }
+  const uint32_t lw = (b / 8u) * 4u + w;  // wave index on this XCD
+  const uint32_t m = A.dxk.div(lw);
+  const uint32_t item = (m * 8u + b % 8u) * A.xk + (lw - m * A.xk);
   if (item >= A.nwaves) return;
   BulkIn<L, S, CPT> in;
   const RowPos r = item_pos(item, A, P, CPT);
@@ -736,7 +729,7 @@ static void dispatch(int q, int prec, F&& fn) {
   }
 }
 
-int cells_per_thread(const KParams& P) { return (P.nx % 2 == 0) ? 2 : 1; }
+int cells_per_thread(const KParams& P) { return (P.cpt == 2 && P.nx % 2 == 0) ? 2 : 1; }
 
 void launch_init(const LaunchCtx& c, void* f) {
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
@@ -774,7 +767,7 @@ void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* m
 }
 
 StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
-                   int xcd) {
+                   int64_t chunk) {
   StepArgs A;
   const int cpt = cells_per_thread(c.P);
   A.wlist = list;
@@ -786,10 +779,14 @@ StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* met
   A.wpr = (uint32_t)((c.P.nx + 64 * cpt - 1) / (64 * cpt));
   A.dwpr.init(A.wpr);
   A.nwaves = (uint32_t)((int64_t)c.P.ny * c.P.nz * A.wpr);
-  A.per = (A.nwaves + 7u) / 8u;
-  // with the XCD placement every XCD gets ceil(per / 4) blocks
-  A.nbb = xcd ? 8u * ((A.per + 3u) / 4u) : (A.nwaves + 3u) / 4u;
-  A.xcd = xcd;
+  // chunk <= 0: one contiguous eighth of the items per XCD
+  const int64_t per = ((int64_t)A.nwaves + 7) / 8;
+  A.xk = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(chunk > 0 ? chunk : per, per));
+  A.dxk.init(A.xk);
+  A.pad = 0;
+  const int64_t runs = ((int64_t)A.nwaves + A.xk - 1) / A.xk;
+  const int64_t waves_per_xcd = (runs + 7) / 8 * (int64_t)A.xk;
+  A.nbb = (uint32_t)(8 * ((waves_per_xcd + 3) / 4));
   return A;
 }
 

@@ -19,8 +19,9 @@ void launch_classify(const LaunchCtx& c, uint32_t* list, uint32_t* masks, unsign
 // Links of the wall cells (direction-major, stride cap) and their meta words.
 void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, WallLink* links, uint32_t* meta, int64_t count,
                      int64_t cap);
+// chunk: items per XCD run (<= 0: one contiguous eighth of the items per XCD)
 StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
-                   int xcd);
+                   int64_t chunk);
 void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A);
 int cells_per_thread(const KParams& P);
 int64_t macro_blocks(int64_t n);

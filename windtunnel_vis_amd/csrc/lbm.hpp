@@ -188,7 +188,9 @@ struct KParams {
   int32_t ground, sideMode, collision, interp;
   float spongeStart, spongeNu, uGround, emaAlpha;
   float uRef, tauWall, spongeIn, contactH;
-  int32_t refX, closure, pad0, pad1;
+  int32_t refX, closure;
+  int32_t cpt;        // cells per lane of the bulk update (2: pairs along x, needs even nx; 1)
+  int32_t pad1;
   float invSponge;    // 1 / max(nx - spongeStart, 1)
   float invSpongeIn;  // 1 / max(spongeIn, 1)
   float smag;         // 18 sqrt(2) Cs^2
@@ -224,9 +226,10 @@ struct StepArgs {
   uint32_t wpr;     // waves per x-row
   uint32_t nwaves;  // bulk items (one wave-sized chunk of an x-row each)
   uint32_t nbb;     // bulk blocks
-  uint32_t per;     // items per XCD, ceil(nwaves / 8)
-  int32_t xcd;      // XCD-aware placement of bulk items
+  uint32_t xk;      // items per XCD chunk (consecutive items run on one XCD)
+  uint32_t pad;
   FastDiv dwpr;     // division by wpr
+  FastDiv dxk;      // division by xk
 };
 
 // wgslLbm.ts:21-28

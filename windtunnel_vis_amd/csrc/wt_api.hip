@@ -66,7 +66,8 @@ struct wt_solver {
   // profiling
   bool prof = false;
   std::vector<hipEvent_t> ev;   // per step: start, stop of the step kernel
-  int xcd_remap = 1;            // WT_TUNE_XCD_REMAP
+  int item_chunk = 0;           // WT_TUNE_ITEM_CHUNK (0: one contiguous eighth per XCD)
+  int cpt = 2;                  // WT_TUNE_CELLS_PER_THREAD (2 needs an even nx)
   size_t ev_used = 0;
   double acc_bulk = 0, acc_bnd = 0;
   int64_t acc_launches = 0;
@@ -115,6 +116,7 @@ void derive(wt_solver* s) {
   P.smag = (float)(18.0 * std::sqrt(2.0) * p.cs * p.cs);
   P.divx.init((uint32_t)s->dims.nx);
   P.divy.init((uint32_t)s->dims.ny);
+  P.cpt = (s->cpt == 2 && s->dims.nx % 2 == 0) ? 2 : 1;
 }
 
 int check_params(const wt_params* p) {
@@ -179,7 +181,7 @@ int do_steps(wt_solver* s, int steps) {
     if (r) return r;
   }
   LaunchCtx c = s->ctx();
-  const StepArgs A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->xcd_remap);
+  const StepArgs A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->item_chunk);
   for (int k = 0; k < steps; k++) {
     const void* fin = s->f[s->parity];
     void* fout = s->f[s->parity ^ 1];
@@ -569,16 +571,109 @@ int wt_copy_probe(int64_t bytes, int32_t iters, double* gbs) {
 
 int wt_set_tuning(wt_solver* s, int32_t key, int32_t value) {
   if (!s) return fail(WT_E_ARG, "null argument");
-  if (key == WT_TUNE_XCD_REMAP) { s->xcd_remap = value ? 1 : 0; return WT_OK; }
-
+  if (key == WT_TUNE_ITEM_CHUNK) {
+    if (value < 0) return fail(WT_E_ARG, "item chunk must be >= 0");
+    s->item_chunk = value;
+    return WT_OK;
+  }
+  if (key == WT_TUNE_CELLS_PER_THREAD) {
+    if (value != 1 && value != 2) return fail(WT_E_ARG, "cells per thread must be 1 or 2");
+    // FP16 / BF16 words hold two cells: those formats need pairs
+    const int v = s->prec != WT_PREC_F32 ? 2 : value;
+    if (v == s->cpt) return WT_OK;
+    s->cpt = v;
+    derive(s);
+    return reclassify(s);  // the wall list is built per group of cells
+  }
   return fail(WT_E_ARG, "unknown tuning key");
 }
 
 int wt_get_tuning(const wt_solver* s, int32_t key, int32_t* value) {
   if (!s || !value) return fail(WT_E_ARG, "null argument");
-  if (key == WT_TUNE_XCD_REMAP) { *value = s->xcd_remap; return WT_OK; }
-
+  if (key == WT_TUNE_ITEM_CHUNK) { *value = s->item_chunk; return WT_OK; }
+  if (key == WT_TUNE_CELLS_PER_THREAD) { *value = s->P.cpt; return WT_OK; }
   return fail(WT_E_ARG, "unknown tuning key");
+}
+
+int wt_autotune(wt_solver* s, int32_t steps, wt_tune_result* out) {
+  if (!s || !out) return fail(WT_E_ARG, "null argument");
+  if (steps < 1) return fail(WT_E_ARG, "steps must be >= 1");
+  if (s->pending_init) {
+    if (int r = run_init(s)) return r;
+  }
+  std::memset(out, 0, sizeof(*out));
+  // candidate runs of items per XCD: the plain order, one x-row, 8 rows, one z-plane, 4 planes,
+  // one contiguous eighth per XCD
+  const int cpt0 = s->P.cpt;
+  const int64_t wpr = (s->dims.nx + 64 * cpt0 - 1) / (64 * cpt0);
+  const int64_t cands[] = {4, wpr, 8 * wpr, wpr * s->dims.ny, 4 * wpr * s->dims.ny, 0};
+  // the trials step the solver: keep its state and restore it afterwards
+  const int64_t sbytes = s->prec == WT_PREC_F32 ? 4 : 2;
+  const size_t fbytes = (size_t)((int64_t)s->q * s->n * sbytes);
+  void* keep = nullptr;
+  if (hipMalloc(&keep, fbytes) != hipSuccess) return fail(WT_E_NOMEM, "autotune snapshot allocation failed");
+  const int parity0 = s->parity;
+  const int64_t step0 = s->step;
+  const bool prof0 = s->prof;
+  const int chunk0 = s->item_chunk;
+  int rc = WT_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto run = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(keep, s->f[parity0], fbytes, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    s->prof = false;
+    double best = 1e30;
+    int n = 0;
+    for (int cpt : {2, 1}) {
+      if (cpt == 2 && (s->dims.nx % 2 != 0)) continue;
+      if (cpt == 1 && s->prec != WT_PREC_F32) continue;  // 16-bit words hold pairs
+      if (int r = wt_set_tuning(s, WT_TUNE_CELLS_PER_THREAD, cpt)) return r;
+      for (int64_t ck : cands) {
+        if (n >= WT_TUNE_MAX_TRIALS) break;
+        s->item_chunk = (int)ck;
+        if (int r = do_steps(s, 2)) return r;  // warm-up
+        HIP_TRY(hipEventRecord(e0, s->stream));
+        if (int r = do_steps(s, steps)) return r;
+        HIP_TRY(hipEventRecord(e1, s->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / steps;
+        out->trial_chunk[n] = (int32_t)ck;
+        out->trial_cpt[n] = cpt;
+        out->trial_ms[n] = per;
+        n++;
+        if (per < best) {
+          best = per;
+          out->chunk = (int32_t)ck;
+          out->cells_per_thread = cpt;
+          out->ms_per_step = per;
+        }
+      }
+    }
+    out->trials = n;
+    return WT_OK;
+  };
+  rc = run();
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc == WT_OK) rc = wt_set_tuning(s, WT_TUNE_CELLS_PER_THREAD, out->cells_per_thread);
+  if (rc == WT_OK) {
+    s->item_chunk = out->chunk;
+  } else {
+    s->item_chunk = chunk0;
+  }
+  // restore the state the trials stepped away from
+  const hipError_t e = hipMemcpyAsync(s->f[parity0], keep, fbytes, hipMemcpyDeviceToDevice, s->stream);
+  const hipError_t e2 = hipStreamSynchronize(s->stream);
+  (void)hipFree(keep);
+  s->parity = parity0;
+  s->step = step0;
+  s->prof = prof0;
+  if (rc != WT_OK) return rc;
+  if (e != hipSuccess || e2 != hipSuccess) return fail(WT_E_HIP, "autotune state restore failed");
+  return WT_OK;
 }
 
 int wt_work_split(const wt_solver* s, int64_t* bulk_cells, int64_t* boundary_cells) {

@@ -245,6 +245,15 @@ class SolverGPU:
         N.check(self._lib.wt_get_tuning(self._h, int(key), N.C.byref(v)))
         return v.value
 
+    def autotune(self, steps=20):
+        """Time every candidate launch configuration for `steps` steps, keep the fastest (the state
+        is restored afterwards); returns {chunk, cellsPerThread, msPerStep, trials: [...]}."""
+        r = N.TuneResult()
+        N.check(self._lib.wt_autotune(self._h, int(steps), N.C.byref(r)))
+        trials = [dict(chunk=r.trial_chunk[k], cellsPerThread=r.trial_cpt[k], msPerStep=r.trial_ms[k])
+                  for k in range(r.trials)]
+        return dict(chunk=r.chunk, cellsPerThread=r.cells_per_thread, msPerStep=r.ms_per_step, trials=trials)
+
     def workSplit(self):
         a = N.C.c_int64()
         b = N.C.c_int64()
