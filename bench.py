@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--north-star", action="store_true", help="also time 512x256x256")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: copy probe + the main workload only")
+    ap.add_argument("--chunk", type=int, default=None,
+                    help="fixed item-run length instead of the autotuner (4 = plain order, 0 = eighths)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -160,7 +162,7 @@ def main():
         args.no_physics = args.no_cpu = True
         args.north_star = False
     copy_gbs = copy_probe(256 << 20, 3) if args.profile_run else copy_probe()
-    r = run_gpu(dims, args.steps, args.warmup, args.precision)
+    r = run_gpu(dims, args.steps, args.warmup, args.precision, chunk=args.chunk)
     out = {
         "metric": METRIC, "value": round(r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 5), "higher_is_better": True,
@@ -199,7 +201,7 @@ def main():
         rb["solver"].destroy()
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        steps = args.cpu_steps or 20
+        steps = args.cpu_steps or 100
         cb = cpu_baseline(dims, steps, threads)
         out["cpu_baseline"] = {"value": round(cb["value"], 2), "unit": "MLUPS", "cores": threads, "kind": "port",
                                "sample": "%d steps of the same %dx%dx%d sedan domain (FP32 CPU restatement, FP16 store "
