@@ -117,6 +117,74 @@ def physics(s, body, info, u, total=5000, chunk=250):
     return dict(cd=acc_d / max(k, 1), cl=acc_l / max(k, 1), unstable=unstable, frontal=info["frontal"])
 
 
+def sweep_bench(steps=500, total=2000, chunk=250):
+    """Batched yaw / ride-height sweep: 24 uneven D3Q19 FP16 domains (autotuned launch configuration
+    per domain) stepped by one launch per step in 100-step hipGraph chunks. Aggregate MLUPS (batched
+    with graphs, batched direct launches, and the same domains stepped one after another), then
+    per-case C_D / C_L over the second half of `total` steps."""
+    from windtunnel_vis_amd import BatchGPU, sweep, tuning
+    from windtunnel_vis_amd.analysis import coefficients
+    cases = sweep.sweep_cases()
+    made = sweep.make_solvers(cases)
+    solvers = [m[0] for m in made]
+    for s_ in solvers:  # launch configuration per domain from the autotuner
+        tuning.autotune(s_, steps=10)
+    cells = sum(s.n for s in solvers)
+    batch = BatchGPU(solvers, chunk=100, graph=True)
+    batch.run(100)
+    batch.sync()
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    def run_batch():
+        batch.run(steps)
+        batch.sync()
+
+    t_graph = timed(run_batch)
+    batch.setChunk(100, graph=False)
+    t_direct = timed(run_batch)
+    batch.setChunk(100, graph=True)
+
+    def run_serial():
+        for s_ in solvers:
+            s_.step(steps)
+        for s_ in solvers:
+            s_.sync()
+
+    batch.destroy()
+    t_serial = timed(run_serial)
+    batch = BatchGPU(solvers, chunk=100, graph=True)
+    for s_ in solvers:
+        s_.reset()
+    u = sweep.sweep_params()["u"]
+    acc = [[0.0, 0.0, 0] for _ in solvers]
+    unstable = [False] * len(solvers)
+    for k in range(0, total, chunk):
+        batch.run(chunk)
+        st = batch.stats()
+        if k + chunk > total // 2:
+            for i, (r, (s_, body, info)) in enumerate(zip(st, made)):
+                c = coefficients(r.fx, r.fy, r.fz, u, info["frontal"])
+                acc[i][0] += c["cd"]
+                acc[i][1] += c["cl"]
+                acc[i][2] += 1
+                unstable[i] |= r.unstable
+    batch.destroy()
+    per_case = [{"dims": list(c["dims"]), "body": c["kind"], "yaw": c["yaw"], "ride": c["ride"],
+                 "cd": round(a[0] / max(a[2], 1), 4), "cl": round(a[1] / max(a[2], 1), 4), "unstable": u_}
+                for c, a, u_ in zip(cases, acc, unstable)]
+    for s_ in solvers:
+        s_.destroy()
+    return {"domains": len(cases), "cells": cells, "steps": steps,
+            "mlups_batched_graph": round(cells * steps / t_graph / 1e6, 1),
+            "mlups_batched_direct": round(cells * steps / t_direct / 1e6, 1),
+            "mlups_serial_per_domain": round(cells * steps / t_serial / 1e6, 1),
+            "cd_steps": total, "cases": per_case}
+
+
 def cpu_baseline(dims, sample_steps, threads):
     """The CPU restatement (FP32 arithmetic path, OpenMP) on the same domain for a few steps."""
     from oracle.oracle import OracleSolver
@@ -149,6 +217,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU-baseline sample steps (0 = auto)")
     ap.add_argument("--north-star", action="store_true", help="also time 512x256x256")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched 24-domain sweep")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: copy probe + the main workload only")
     ap.add_argument("--chunk", type=int, default=None,
@@ -159,7 +228,7 @@ def main():
         sys.exit(2)
     dims = tuple(int(v) for v in args.dims.split(","))
     if args.profile_run:
-        args.no_physics = args.no_cpu = True
+        args.no_physics = args.no_cpu = args.no_sweep = True
         args.north_star = False
     copy_gbs = copy_probe(256 << 20, 3) if args.profile_run else copy_probe()
     r = run_gpu(dims, args.steps, args.warmup, args.precision, chunk=args.chunk)
@@ -199,6 +268,8 @@ def main():
                                           "kernel_frac": round(rb["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                           "avg_launch_ms": round(rb["k_avg_ms"], 5)}
         rb["solver"].destroy()
+    if not args.no_sweep:
+        out["batched_sweep"] = sweep_bench()
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         steps = args.cpu_steps or 100

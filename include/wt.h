@@ -249,6 +249,25 @@ typedef struct wt_tune_result {
 } wt_tune_result;
 int wt_autotune(wt_solver* s, int32_t steps, wt_tune_result* out);
 
+/* ---- batched sweeps (no reference counterpart: the reference runs a study's cases one after
+ * another, studies.ts:223-252) ------------------------------------------------------------------
+ * A batch steps many independent solver handles (e.g. a yaw / ride-height sweep of uneven grids)
+ * with ONE kernel launch per step: a device descriptor table gives every block its domain's
+ * parameters, layout and buffers. Steps run as replays of a hipGraph-captured chunk (default 100
+ * steps) plus direct launches for the remainder. The members must share lattice, storage
+ * precision, cells per thread and collision, must have been stepped equally often, and stay
+ * owned by the caller (destroy the batch first). Geometry, parameter and tuning changes on a
+ * member are picked up at the next wt_batch_run. Per-member statistics are read with
+ * wt_batch_stats (one wt_stats per member, in member order). */
+typedef struct wt_batch wt_batch;
+int wt_batch_create(wt_solver* const* members, int32_t count, wt_batch** out);
+int wt_batch_destroy(wt_batch* b);
+/* chunk_steps >= 2 (rounded up to even); use_graph = 0 launches every step directly. */
+int wt_batch_set_chunk(wt_batch* b, int32_t chunk_steps, int32_t use_graph);
+int wt_batch_run(wt_batch* b, int32_t steps);
+int wt_batch_sync(wt_batch* b);
+int wt_batch_stats(wt_batch* b, wt_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

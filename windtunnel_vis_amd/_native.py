@@ -113,6 +113,12 @@ SIGNATURES = {
     "wt_set_tuning": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "wt_get_tuning": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "wt_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(TuneResult)]),
+    "wt_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
+    "wt_batch_destroy": (C.c_int, [_P]),
+    "wt_batch_set_chunk": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "wt_batch_run": (C.c_int, [_P, C.c_int32]),
+    "wt_batch_sync": (C.c_int, [_P]),
+    "wt_batch_stats": (C.c_int, [_P, C.POINTER(Stats)]),
 }
 
 _lib = None

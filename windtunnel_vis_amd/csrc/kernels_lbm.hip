@@ -482,14 +482,8 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
 template <class S> constexpr int step_waves() { return S::BYTES == 2 ? 6 : 4; }
 
 template <class L, class S, int CPT, bool REG>
-__global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                                const uint8_t* __restrict__ kind, KParams P, StepArgs A) {
-  uint32_t b = blockIdx.x;
-  if (b < A.nwb) {
-    wall_cell<L, S>(fin, fout, A, P, (int64_t)b * BLOCK + threadIdx.x);
-    return;
-  }
-  b -= A.nwb;
+__device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                           const uint8_t* __restrict__ kind, const KParams& P, const StepArgs& A, uint32_t b) {
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lw = (b / 8u) * 4u + w;  // wave index on this XCD
   const uint32_t m = A.dxk.div(lw);
@@ -501,6 +495,36 @@ __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename 
   // ground rows relax with BGK (wall cells, wgslLbm.ts:186); the branch is wave-uniform
   if (REG && !(r.iy == 0 && P.ground != 0)) bulk_finish<L, S, CPT, true>(fout, P, r, in);
   else bulk_finish<L, S, CPT, false>(fout, P, r, in);
+}
+
+template <class L, class S, int CPT, bool REG>
+__global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                const uint8_t* __restrict__ kind, KParams P, StepArgs A) {
+  const uint32_t b = blockIdx.x;
+  if (b < A.nwb) {
+    wall_cell<L, S>(fin, fout, A, P, (int64_t)b * BLOCK + threadIdx.x);
+    return;
+  }
+  bulk_block<L, S, CPT, REG>(fin, fout, kind, P, A, b - A.nwb);
+}
+
+// One lattice step of many independent domains in one launch (sweeps): the same wall and bulk
+// work as k_step, each block finding its domain through `dmap` (block -> domain) and its
+// parameters, layout and buffers in the descriptor table. All domains' wall blocks come first.
+template <class L, class S, int CPT, bool REG>
+__global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step_batch(const DomainDesc* __restrict__ D, const uint32_t* __restrict__ dmap,
+                                                                      uint32_t wall_total, int32_t parity) {
+  using T = typename S::T;
+  const uint32_t b = blockIdx.x;
+  const uint32_t d = __builtin_amdgcn_readfirstlane(dmap[b]);
+  const DomainDesc& dd = D[d];
+  const T* fin = (const T*)dd.f[parity];
+  T* fout = (T*)dd.f[parity ^ 1];
+  if (b < wall_total) {
+    wall_cell<L, S>(fin, fout, dd.A, dd.P, (int64_t)(b - dd.wall_first) * BLOCK + threadIdx.x);
+    return;
+  }
+  bulk_block<L, S, CPT, REG>(fin, fout, dd.kind, dd.P, dd.A, b - dd.bulk_first);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -768,7 +792,7 @@ void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* m
 
 StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
                    int64_t chunk) {
-  StepArgs A;
+  StepArgs A{};
   const int cpt = cells_per_thread(c.P);
   A.wlist = list;
   A.wmeta = meta;
@@ -788,6 +812,21 @@ StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* met
   const int64_t waves_per_xcd = (runs + 7) / 8 * (int64_t)A.xk;
   A.nbb = (uint32_t)(8 * ((waves_per_xcd + 3) / 4));
   return A;
+}
+
+void launch_step_batch(hipStream_t st, int q, int prec, int cpt, bool reg, const DomainDesc* descs, const uint32_t* dmap,
+                       uint32_t wall_total, uint32_t blocks, int parity) {
+  dispatch(q, prec, [&](auto l, auto s) {
+    using L = decltype(l); using S = decltype(s);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), 0, st, descs, dmap, wall_total, parity); };
+    if (cpt == 2) {
+      if (reg) go(k_step_batch<L, S, 2, true>);
+      else go(k_step_batch<L, S, 2, false>);
+    } else {
+      if (reg) go(k_step_batch<L, S, 1, true>);
+      else go(k_step_batch<L, S, 1, false>);
+    }
+  });
 }
 
 void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A) {

@@ -23,6 +23,9 @@ void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* m
 StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
                    int64_t chunk);
 void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A);
+// One step of a batch of domains sharing lattice, storage, cells per thread and collision.
+void launch_step_batch(hipStream_t st, int q, int prec, int cpt, bool reg, const DomainDesc* descs, const uint32_t* dmap,
+                       uint32_t wall_total, uint32_t blocks, int parity);
 int cells_per_thread(const KParams& P);
 int64_t macro_blocks(int64_t n);
 void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, float* meanB, double* partials, double* result);

@@ -232,6 +232,17 @@ struct StepArgs {
   FastDiv dxk;      // division by xk
 };
 
+// One domain of a batched step (k_step_batch): its parameters, launch layout, buffers and where its
+// blocks sit in the batched grid (all domains' wall blocks first, then all bulk blocks).
+struct DomainDesc {
+  KParams P;
+  StepArgs A;
+  const void* f[2];
+  const uint8_t* kind;
+  uint32_t wall_first;  // first wall block of the domain
+  uint32_t bulk_first;  // first bulk block of the domain (a multiple of 8)
+};
+
 // wgslLbm.ts:21-28
 __device__ inline float3 wheel_vel(const KWheels& W, const KParams& P, uint32_t k, float3 p) {
   float4 a = W.w[2 * k], b = W.w[2 * k + 1];

@@ -219,6 +219,102 @@ int do_sample(wt_solver* s) {
   return WT_OK;
 }
 
+// SolverGPU.ts:302-308
+void decode_stats(const wt_solver* s, wt_stats* out) {
+  const double* d = s->h_result;
+  out->fx = d[0]; out->fy = d[1]; out->fz = d[2];
+  out->max_u = std::sqrt(std::max(d[3], 0.0));
+  out->rho_ref = d[5] > 0 ? d[4] / d[5] : 1.0;
+  out->unstable = (d[6] > 0 || !std::isfinite(d[0]) || d[3] > 0.25) ? 1 : 0;
+  out->pad = 0;
+  out->step = s->step;
+}
+
+}  // namespace
+
+// A batch of independent domains stepped by one launch per step (k_step_batch).
+struct wt_batch {
+  std::vector<wt_solver*> m;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  DomainDesc* d_desc = nullptr;
+  uint32_t* d_map = nullptr;
+  int64_t map_cap = 0;
+  std::vector<DomainDesc> h_desc;
+  uint32_t wall_total = 0, blocks = 0;
+  int q = 0, prec = 0, cpt = 0;
+  bool reg = true;
+  int chunk = 100;      // steps per captured graph (even)
+  bool use_graph = true;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // per starting parity
+  hipGraph_t graph[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+void batch_drop_graph(wt_batch* b) {
+  for (int p = 0; p < 2; p++) {
+    if (b->gexec[p]) (void)hipGraphExecDestroy(b->gexec[p]);
+    if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
+    b->gexec[p] = nullptr;
+    b->graph[p] = nullptr;
+  }
+}
+
+// Descriptor table and block -> domain map for the members' current state; uploads and drops the
+// captured graph when anything changed.
+int batch_prepare(wt_batch* b) {
+  std::vector<DomainDesc> desc(b->m.size());
+  uint32_t wall = 0;
+  std::vector<uint32_t> nbb(b->m.size());
+  for (size_t k = 0; k < b->m.size(); k++) {
+    wt_solver* s = b->m[k];
+    LaunchCtx c = s->ctx();
+    DomainDesc& d = desc[k];
+    std::memset(&d, 0, sizeof(d));
+    d.P = s->P;
+    d.A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->item_chunk);
+    d.f[0] = s->f[0];
+    d.f[1] = s->f[1];
+    d.kind = s->kind;
+    d.wall_first = wall;
+    wall += d.A.nwb;  // multiples of 8: bulk blocks stay XCD-aligned
+    nbb[k] = d.A.nbb;
+  }
+  uint32_t bulk = wall;
+  for (size_t k = 0; k < b->m.size(); k++) {
+    desc[k].bulk_first = bulk;
+    bulk += nbb[k];
+  }
+  const bool same = desc.size() == b->h_desc.size() &&
+                    std::memcmp(desc.data(), b->h_desc.data(), desc.size() * sizeof(DomainDesc)) == 0;
+  if (same && b->d_desc) return WT_OK;
+  batch_drop_graph(b);
+  std::vector<uint32_t> map(bulk);
+  for (size_t k = 0; k < b->m.size(); k++) {
+    for (uint32_t i = 0; i < desc[k].A.nwb; i++) map[desc[k].wall_first + i] = (uint32_t)k;
+    for (uint32_t i = 0; i < nbb[k]; i++) map[desc[k].bulk_first + i] = (uint32_t)k;
+  }
+  if (!b->d_desc) HIP_TRY(hipMalloc((void**)&b->d_desc, sizeof(DomainDesc) * b->m.size()));
+  if ((int64_t)map.size() > b->map_cap) {
+    if (b->d_map) HIP_TRY(hipFree(b->d_map));
+    b->d_map = nullptr;
+    HIP_TRY(hipMalloc((void**)&b->d_map, sizeof(uint32_t) * map.size()));
+    b->map_cap = (int64_t)map.size();
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_desc, desc.data(), sizeof(DomainDesc) * desc.size(), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_map, map.data(), sizeof(uint32_t) * map.size(), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->h_desc = desc;
+  b->wall_total = wall;
+  b->blocks = bulk;
+  return WT_OK;
+}
+
+void batch_launch(wt_batch* b, int parity) {
+  launch_step_batch(b->stream, b->q, b->prec, b->cpt, b->reg, b->d_desc, b->d_map, b->wall_total, b->blocks, parity);
+}
+
 }  // namespace
 
 extern "C" {
@@ -427,14 +523,7 @@ int wt_step_and_sample(wt_solver* s, int32_t steps, wt_stats* out) {
   if (steps < 0) return fail(WT_E_ARG, "steps must be >= 0");
   if (int r = do_steps(s, steps)) return r;
   if (int r = do_sample(s)) return r;
-  const double* d = s->h_result;
-  // SolverGPU.ts:302-308
-  out->fx = d[0]; out->fy = d[1]; out->fz = d[2];
-  out->max_u = std::sqrt(std::max(d[3], 0.0));
-  out->rho_ref = d[5] > 0 ? d[4] / d[5] : 1.0;
-  out->unstable = (d[6] > 0 || !std::isfinite(d[0]) || d[3] > 0.25) ? 1 : 0;
-  out->pad = 0;
-  out->step = s->step;
+  decode_stats(s, out);
   return WT_OK;
 }
 
@@ -722,6 +811,113 @@ int wt_voxelize_mesh(wt_solver* s, const float* positions, int32_t n_vertices, c
       info->min[a] = h[1] > 0 ? (int32_t)h[2 + a] : 0;
       info->max[a] = h[1] > 0 ? (int32_t)h[5 + a] + 1 : 0;
     }
+  }
+  return WT_OK;
+}
+
+// ---- batched sweeps ----------------------------------------------------------------------
+
+int wt_batch_create(wt_solver* const* members, int32_t count, wt_batch** out) {
+  if (!members || !out || count < 1) return fail(WT_E_ARG, "bad batch member list");
+  *out = nullptr;
+  const wt_solver* m0 = members[0];
+  if (!m0) return fail(WT_E_ARG, "null batch member");
+  for (int32_t k = 0; k < count; k++) {
+    const wt_solver* s = members[k];
+    if (!s) return fail(WT_E_ARG, "null batch member");
+    if (s->q != m0->q || s->prec != m0->prec || s->P.cpt != m0->P.cpt || (s->P.collision != 1) != (m0->P.collision != 1))
+      return fail(WT_E_ARG, "batch members must share lattice, storage precision, cells per thread and collision");
+    if (s->parity != m0->parity) return fail(WT_E_STATE, "batch members must have been stepped equally often (same parity)");
+    if (s->device != m0->device) return fail(WT_E_ARG, "batch members must live on one device");
+    for (int32_t j = 0; j < k; j++)
+      if (members[j] == s) return fail(WT_E_ARG, "a solver appears twice in the batch");
+  }
+  auto* b = new wt_batch();
+  b->m.assign(members, members + count);
+  b->q = m0->q;
+  b->prec = m0->prec;
+  b->cpt = m0->P.cpt;
+  b->reg = m0->P.collision != 1;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    wt_batch_destroy(b);
+    return fail(WT_E_HIP, "batch stream creation failed");
+  }
+  *out = b;
+  return WT_OK;
+}
+
+int wt_batch_destroy(wt_batch* b) {
+  if (!b) return WT_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_map) (void)hipFree(b->d_map);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return WT_OK;
+}
+
+int wt_batch_set_chunk(wt_batch* b, int32_t chunk_steps, int32_t use_graph) {
+  if (!b || chunk_steps < 2) return fail(WT_E_ARG, "chunk must be >= 2 steps");
+  if (b->stream) HIP_TRY(hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
+  b->chunk = (chunk_steps + 1) / 2 * 2;  // even: a replay leaves the parity unchanged
+  b->use_graph = use_graph != 0;
+  return WT_OK;
+}
+
+int wt_batch_run(wt_batch* b, int32_t steps) {
+  if (!b) return fail(WT_E_ARG, "null argument");
+  if (steps < 0) return fail(WT_E_ARG, "steps must be >= 0");
+  const int parity = b->m[0]->parity;
+  for (wt_solver* s : b->m) {
+    if (s->parity != parity) return fail(WT_E_STATE, "batch members were stepped apart (parity differs)");
+    if (s->pending_init) {
+      if (int r = run_init(s)) return r;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));  // uploads, preparation and init are done
+  }
+  if (int r = batch_prepare(b)) return r;
+  int done = 0;
+  if (b->use_graph && steps >= b->chunk) {
+    if (!b->gexec[parity]) {
+      // one captured chunk of `chunk` launches, parities alternating from the current one; chunk
+      // is even, so every replay starts from the same parity
+      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < b->chunk; k++) batch_launch(b, (parity + k) & 1);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+      if (ec != hipSuccess) return fail(WT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      b->graph[parity] = g;
+      HIP_TRY(hipGraphInstantiate(&b->gexec[parity], g, nullptr, nullptr, 0));
+    }
+    for (; done + b->chunk <= steps; done += b->chunk) HIP_TRY(hipGraphLaunch(b->gexec[parity], b->stream));
+  }
+  for (; done < steps; done++) batch_launch(b, (parity + done) & 1);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(b->done, b->stream));
+  for (wt_solver* s : b->m) {
+    HIP_TRY(hipStreamWaitEvent(s->stream, b->done, 0));  // later work on a member follows the batch
+    s->parity = (s->parity + steps) & 1;
+    s->step += steps;
+  }
+  return WT_OK;
+}
+
+int wt_batch_sync(wt_batch* b) {
+  if (!b) return fail(WT_E_ARG, "null argument");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return WT_OK;
+}
+
+int wt_batch_stats(wt_batch* b, wt_stats* out) {
+  if (!b || !out) return fail(WT_E_ARG, "null argument");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t k = 0; k < b->m.size(); k++) {
+    if (int r = do_sample(b->m[k])) return r;
+    decode_stats(b->m[k], &out[k]);
   }
   return WT_OK;
 }

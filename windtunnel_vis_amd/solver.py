@@ -270,3 +270,43 @@ def wheel_array(wheels):
 
 def voxel_info(vi):
     return dict(frontal=int(vi.frontal), solidCells=int(vi.solid_cells), min=list(vi.min), max=list(vi.max))
+
+
+class BatchGPU:
+    """Many independent SolverGPU domains stepped by one launch per step (wt_batch_*), in
+    hipGraph-captured chunks. The solvers stay owned by the caller; destroy the batch first."""
+
+    def __init__(self, solvers, chunk=100, graph=True):
+        self._lib = N.load()
+        self.solvers = list(solvers)
+        arr = (N.C.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = N.C.c_void_p()
+        N.check(self._lib.wt_batch_create(arr, len(self.solvers), N.C.byref(h)))
+        self._h = h
+        self.setChunk(chunk, graph)
+
+    def setChunk(self, chunk, graph=True):
+        N.check(self._lib.wt_batch_set_chunk(self._h, int(chunk), 1 if graph else 0))
+
+    def run(self, steps):
+        N.check(self._lib.wt_batch_run(self._h, int(steps)))
+
+    def sync(self):
+        N.check(self._lib.wt_batch_sync(self._h))
+
+    def stats(self):
+        """SolverStats per member (synchronises)."""
+        arr = (N.Stats * len(self.solvers))()
+        N.check(self._lib.wt_batch_stats(self._h, arr))
+        return [SolverStats(a.fx, a.fy, a.fz, a.max_u, a.rho_ref, bool(a.unstable), int(a.step)) for a in arr]
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            self._lib.wt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
