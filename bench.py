@@ -268,6 +268,14 @@ def main():
                                           "kernel_frac": round(rb["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                           "avg_launch_ms": round(rb["k_avg_ms"], 5)}
         rb["solver"].destroy()
+    if not args.no_physics:
+        from windtunnel_vis_amd.validation import config1_cylinder
+        from windtunnel_vis_amd import SolverGPU
+        c1 = config1_cylinder(lambda d, p: SolverGPU(d, p))
+        rd = lambda w: {"St": round(w["St"], 4) if w["St"] == w["St"] else None, "cd": round(w["cd"], 4),
+                        "cycles": w["cycles"], "steps": w["steps"]}
+        out["config1_cylinder"] = {"window_20k": rd(c1["window_20k"]), "developed": rd(c1["developed"]),
+                                   "unstable": c1["unstable"], "seconds": round(c1["seconds"], 2)}
     if not args.no_sweep:
         out["batched_sweep"] = sweep_bench()
     if not args.no_cpu:
