@@ -268,6 +268,59 @@ int wt_batch_run(wt_batch* b, int32_t steps);
 int wt_batch_sync(wt_batch* b);
 int wt_batch_stats(wt_batch* b, wt_stats* out);
 
+/* ---- learned subgrid closure (no reference counterpart; it replaces the Smagorinsky tau of
+ * wgslLbm.ts:214 when params.closure = WT_CLOSURE_MLP) --------------------------------------------
+ * A per-cell MLP 8-32-32-1 (ReLU hidden layers, softplus output) predicts the eddy viscosity nu_t;
+ * the cell relaxes with tau = t0 + 3 nu_t (t0: tau0 + outlet sponge, ground-wall floor). Inputs:
+ * six rotation invariants of the strain rate S = -3 Pi^neq / (2 rho tau_smag) (tr S, the second
+ * and third principal invariants, tr S^2, tr S^3, |S|; each compressed as sign(x) log(1 + |x| /
+ * 1e-3^degree)), the wall distance (cells to the nearest solid cell centre, capped at 4) and the
+ * local Reynolds number |u| max(d, 1/2) / nu. Inference runs in the step kernel on BF16 MFMA
+ * (FP32 accumulation) for bulk cells; cells next to the body and non-finite outputs keep
+ * Smagorinsky, which is also the closure when no network is loaded. D3Q19 only. */
+#define WT_MLP_NPARAM 1377
+typedef struct wt_mlp {
+  float params[WT_MLP_NPARAM]; /* W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3 (out x in, row-major) */
+  float mu[8];                 /* input normalisation: (x - mu) / sigma */
+  float sigma[8];
+  float nu_scale;              /* nu_t = nu_scale * softplus(z) */
+} wt_mlp;
+int wt_mlp_load(wt_solver* s, const wt_mlp* net);
+/* Wall distance per cell as bytes (32 per cell, capped at 128). */
+int wt_read_wall_distance(wt_solver* s, uint8_t* out);
+/* nu_t of n cells from their raw (unnormalised) inputs (n x 8), on the step kernel's MFMA path. */
+int wt_mlp_infer(const wt_mlp* net, const float* raw, int64_t n, float* nu);
+
+/* Training data: records of 10 floats (8 raw inputs, target nu_t, coarse cell index as bits) in
+ * device memory. wt_dataset_add box-filters a fine D3Q19 run 2x onto the grid of `coarse` (same
+ * geometry at half resolution; it supplies the wall distances and nu): per coarse cell the block
+ * mean velocity, the subgrid stress tau_ij = <u_i u_j> - <u_i><u_j>, the filtered strain rate by
+ * central differences and the least-squares eddy viscosity nu_t = max(0, -tau_dev:S / (2 S:S));
+ * cells touching a solid or a domain face are skipped. Records are appended in cell order. */
+typedef struct wt_dataset wt_dataset;
+int wt_dataset_create(int64_t capacity, wt_dataset** out);
+int wt_dataset_destroy(wt_dataset* d);
+int wt_dataset_add(wt_dataset* d, wt_solver* fine, wt_solver* coarse, int64_t* added);
+int wt_dataset_count(const wt_dataset* d, int64_t* n);
+int wt_dataset_read(const wt_dataset* d, int64_t first, int64_t count, float* out);
+int wt_dataset_append(wt_dataset* d, const float* records, int64_t count);
+
+/* GPU training with hand-written kernels (FP32 forward / backward, FP64 gradient reduction, Adam):
+ * loss = mean (softplus(z) - min(nu_t / nu_scale, 100))^2 over mini-batches of `batch` records in
+ * a fresh random order every epoch (`max_samples` > 0 limits the records drawn per epoch).
+ * net->params is the initial state on entry and the trained state on return; mu, sigma and
+ * nu_scale are used as given. epoch_loss (epochs entries) may be NULL. */
+typedef struct wt_train_opts {
+  int32_t epochs;
+  int32_t batch;
+  double lr, beta1, beta2, eps;
+  uint64_t seed;
+  int64_t max_samples;
+} wt_train_opts;
+int wt_mlp_train(wt_dataset* d, wt_mlp* net, const wt_train_opts* o, double* epoch_loss);
+/* Loss and its gradient over records [first, first + count) (one mini-batch, for checking). */
+int wt_mlp_grad(wt_dataset* d, const wt_mlp* net, int64_t first, int64_t count, float* grad, double* loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,18 @@ class TuneResult(C.Structure):
                 ("trial_cpt", C.c_int32 * 16), ("trial_ms", C.c_double * 16)]
 
 
+MLP_NPARAM = 1377
+
+
+class Mlp(C.Structure):
+    _fields_ = [("params", C.c_float * MLP_NPARAM), ("mu", C.c_float * 8), ("sigma", C.c_float * 8), ("nu_scale", C.c_float)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_double), ("beta1", C.c_double),
+                ("beta2", C.c_double), ("eps", C.c_double), ("seed", C.c_uint64), ("max_samples", C.c_int64)]
+
+
 class Prim(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("flags", C.c_int32), ("center", C.c_float * 3), ("rot", C.c_float * 9),
@@ -119,6 +131,17 @@ SIGNATURES = {
     "wt_batch_run": (C.c_int, [_P, C.c_int32]),
     "wt_batch_sync": (C.c_int, [_P]),
     "wt_batch_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "wt_mlp_load": (C.c_int, [_P, C.POINTER(Mlp)]),
+    "wt_read_wall_distance": (C.c_int, [_P, C.c_void_p]),
+    "wt_mlp_infer": (C.c_int, [C.POINTER(Mlp), C.c_void_p, C.c_int64, C.c_void_p]),
+    "wt_dataset_create": (C.c_int, [C.c_int64, C.POINTER(_P)]),
+    "wt_dataset_destroy": (C.c_int, [_P]),
+    "wt_dataset_add": (C.c_int, [_P, _P, _P, C.POINTER(C.c_int64)]),
+    "wt_dataset_count": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "wt_dataset_read": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_void_p]),
+    "wt_dataset_append": (C.c_int, [_P, C.c_void_p, C.c_int64]),
+    "wt_mlp_train": (C.c_int, [_P, C.POINTER(Mlp), C.POINTER(TrainOpts), C.c_void_p]),
+    "wt_mlp_grad": (C.c_int, [_P, C.POINTER(Mlp), C.c_int64, C.c_int64, C.c_void_p, C.POINTER(C.c_double)]),
 }
 
 _lib = None

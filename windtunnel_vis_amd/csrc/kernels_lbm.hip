@@ -274,7 +274,8 @@ template <class L, class S, int CPT>
 struct BulkIn {
   using Raw = typename std::conditional<CPT == 2, typename S::Word, typename S::T>::type;
   Raw raw[L::Q];
-  uint32_t k;  // kind byte(s) of the lane's cells
+  uint32_t k;   // kind byte(s) of the lane's cells
+  uint32_t wd;  // wall-distance byte(s) of the lane's cells (learned closure only)
 };
 
 // Row coordinates of an item (wave-uniform).
@@ -294,9 +295,10 @@ __device__ __forceinline__ RowPos item_pos(uint32_t item, const StepArgs& A, con
   return r;
 }
 
-template <class L, class S, int CPT>
+template <class L, class S, int CPT, bool MLP = false>
 __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin, const uint8_t* __restrict__ kind,
-                                          const KParams& P, const RowPos& r, BulkIn<L, S, CPT>& in) {
+                                          const KParams& P, const RowPos& r, BulkIn<L, S, CPT>& in,
+                                          const uint8_t* __restrict__ wdist = nullptr) {
   using T = typename S::T;
   constexpr int Q = L::Q;
   constexpr bool D3 = L::DIM == 3;
@@ -310,6 +312,10 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
   // the skip test
   if constexpr (CPT == 2) in.k = *(const uint16_t*)(kind + c0);
   else in.k = kind[c0];
+  if constexpr (MLP) {
+    if constexpr (CPT == 2) in.wd = *(const uint16_t*)(wdist + c0);
+    else in.wd = wdist[c0];
+  }
   const bool yLo = iy == 0, yHi = iy == NY - 1;
   const bool zLo = D3 && iz == 0, zHi = D3 && iz == NZ - 1;
   // per direction: source direction, row and x shift (wave-uniform, branch-free)
@@ -442,6 +448,118 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
   }
 }
 
+// Bulk finish with the learned closure (D3Q19): the moments of both cells are formed with the
+// Smagorinsky tau as a first estimate, the strain rate S = -3 Pi^neq / (2 rho tau) and the other
+// inputs of every cell of the wave go through the network on the MFMA path (all 64 lanes take
+// part, so the skip tests come after it), and each cell relaxes with tau = t0 + 3 nu_t; a
+// non-finite nu_t keeps Smagorinsky.
+template <class L, class S, int CPT, bool REG>
+__device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout, const KParams& P, const RowPos& r,
+                                                const BulkIn<L, S, CPT>& in, const MlpDev* __restrict__ net, short* feats,
+                                                float* nus) {
+  using T = typename S::T;
+  constexpr int Q = L::Q;
+  const int NX = P.nx;
+  const int x0 = r.x0;
+  const int64_t n = P.n;
+  const int64_t rowoff = (int64_t)r.row * NX;
+  const uint32_t k = in.k;
+  constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
+  constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
+  const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
+  auto store = [&](int i, float a, float b) {
+    T* o = (T*)((char*)(fout + uniform_i64((int64_t)i * n + rowoff)) + lane_off);
+    if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
+    else S::st_dev(o, 0, L::w(i), a);
+  };
+  const bool yLo = r.iy == 0;
+  const bool gwall = yLo && P.ground != 0;
+  const bool reg = REG && !gwall;
+  const float U = P.uin;
+  const float beltA = (P.ground == 2 && !(k & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
+  const float beltB = (P.ground == 2 && !((k >> 8) & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
+  auto pull_a = [&](int i) -> float {
+    const int cx = L::c(i, 0), cy = L::c(i, 1);
+    float v;
+    if constexpr (CPT == 2) v = S::lo_dev(in.raw[i], L::w(i));
+    else v = S::dev_of(in.raw[i], L::w(i));
+    if (i > 0) {
+      const bool grnd = cy > 0 && yLo && P.ground != 0;
+      if (grnd && cx != 0) v += beltA * ((6.0f * L::w(i) * cx) * P.uGround);
+      if (cx > 0 && x0 == 0) v = inlet_dev<L>(i, U);
+    }
+    return v;
+  };
+  auto pull_b = [&](int i) -> float {
+    const int cx = L::c(i, 0), cy = L::c(i, 1);
+    const bool grnd = cy > 0 && yLo && P.ground != 0;
+    const bool clamp = i > 0 && cx < 0 && !grnd && x0 + 2 == NX;
+    float v = clamp ? S::lo_dev(in.raw[i], L::w(i)) : S::hi_dev(in.raw[i], L::w(i));
+    if (grnd && cx != 0) v += beltB * ((6.0f * L::w(i) * cx) * P.uGround);
+    return v;
+  };
+  Relax ma, mb;
+  {
+    float f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) f[i] = pull_a(i);
+    ma = relax<L>(f, (float)x0, gwall, reg, P);
+  }
+  if constexpr (CPT == 2) {
+    float f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) f[i] = pull_b(i);
+    mb = relax<L>(f, (float)(x0 + 1), gwall, reg, P);
+  }
+  // inputs of both cells, normalised, staged as BF16 for the wave's MFMA evaluation
+  const int lane = threadIdx.x & 63;
+  const bool live = x0 < NX && k != SOLID_ALL && !(k & SKIP);
+  const float nu0 = (P.tau0 - 0.5f) * (1.0f / 3.0f);
+  auto stage = [&](const Relax& m, uint32_t wdb, int slot) {
+    const float irho = 4.5f * fast_rcp(m.rho45);
+    const float sc = -1.5f * m.omega * irho;  // S = -3 Pi / (2 rho tau)
+    float x[MLP_IN];
+    mlp_raw_features(sc * m.pxx, sc * m.pyy, sc * m.pzz, sc * m.pxy, sc * m.pxz, sc * m.pyz, (float)wdb * (1.0f / 32.0f),
+                     __builtin_sqrtf(m.ux * m.ux + m.uy * m.uy + m.uz * m.uz), nu0, x);
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) {
+      const float v = live ? (x[i] - net->norm.mu[i]) * net->norm.isig[i] : 0.0f;
+      feats[slot * MLP_IN + i] = to_bf16(v == v ? v : 0.0f);
+    }
+  };
+  stage(ma, in.wd & 0xffu, 2 * lane);
+  if constexpr (CPT == 2) stage(mb, (in.wd >> 8) & 0xffu, 2 * lane + 1);
+  else stage(ma, 0u, 2 * lane + 1);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  mlp_wave_nu(net, feats, nus);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const float nua = nus[2 * lane], nub = nus[2 * lane + 1];
+  if (nua == nua && nua < 1e3f) retau(ma, ma.t0 + 3.0f * nua, reg);
+  if constexpr (CPT == 2) {
+    if (nub == nub && nub < 1e3f) retau(mb, mb.t0 + 3.0f * nub, reg);
+  }
+  if (x0 >= NX) return;
+  if (k == SOLID_ALL) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) store(i, 0.0f, 0.0f);
+    return;
+  }
+  if (k & SKIP) return;
+  store(0, rest_out<L>(ma, reg, pull_a(0), P), CPT == 2 ? rest_out<L>(mb, reg, pull_b(0), P) : 0.0f);
+#pragma unroll
+  for (int i = 1; i < Q; i++) {
+    const int o = opp<L>(i);
+    if (o < i) continue;
+    float ai, ao, bi = 0.0f, bo = 0.0f;
+    pair_out<L>(ma, reg, i, pull_a(i), pull_a(o), ai, ao, P);
+    if constexpr (CPT == 2) pair_out<L>(mb, reg, i, pull_b(i), pull_b(o), bi, bo, P);
+    store(i, ai, bi);
+    store(o, ao, bo);
+  }
+}
+
 // Wall cells: one lane per listed cell (cells next to the body). Every pulled population is a
 // precomputed link (two independent gathers and two FMAs), so all loads of a lane are in flight
 // together; then the collision (BGK on wall cells, wgslLbm.ts:186).
@@ -481,9 +599,11 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
 // order (block b takes items 4b..4b+3); A.xk = nwaves / 8 gives every XCD one contiguous eighth.
 template <class S> constexpr int step_waves() { return S::BYTES == 2 ? 6 : 4; }
 
-template <class L, class S, int CPT, bool REG>
+template <class L, class S, int CPT, bool REG, bool MLP = false>
 __device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                           const uint8_t* __restrict__ kind, const KParams& P, const StepArgs& A, uint32_t b) {
+                                           const uint8_t* __restrict__ kind, const KParams& P, const StepArgs& A, uint32_t b,
+                                           const uint8_t* __restrict__ wdist = nullptr, const MlpDev* __restrict__ net = nullptr,
+                                           short* feats = nullptr, float* nus = nullptr) {
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lw = (b / 8u) * 4u + w;  // wave index on this XCD
   const uint32_t m = A.dxk.div(lw);
@@ -491,10 +611,14 @@ __device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin
   if (item >= A.nwaves) return;
   BulkIn<L, S, CPT> in;
   const RowPos r = item_pos(item, A, P, CPT);
-  bulk_load<L, S, CPT>(fin, kind, P, r, in);
-  // ground rows relax with BGK (wall cells, wgslLbm.ts:186); the branch is wave-uniform
-  if (REG && !(r.iy == 0 && P.ground != 0)) bulk_finish<L, S, CPT, true>(fout, P, r, in);
-  else bulk_finish<L, S, CPT, false>(fout, P, r, in);
+  bulk_load<L, S, CPT, MLP>(fin, kind, P, r, in, wdist);
+  if constexpr (MLP) {
+    bulk_finish_mlp<L, S, CPT, REG>(fout, P, r, in, net, feats, nus);
+  } else {
+    // ground rows relax with BGK (wall cells, wgslLbm.ts:186); the branch is wave-uniform
+    if (REG && !(r.iy == 0 && P.ground != 0)) bulk_finish<L, S, CPT, true>(fout, P, r, in);
+    else bulk_finish<L, S, CPT, false>(fout, P, r, in);
+  }
 }
 
 template <class L, class S, int CPT, bool REG>
@@ -506,6 +630,22 @@ __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename 
     return;
   }
   bulk_block<L, S, CPT, REG>(fin, fout, kind, P, A, b - A.nwb);
+}
+
+// The step with the learned closure in the bulk (D3Q19); wall cells keep Smagorinsky.
+template <class S, int CPT, bool REG>
+__global__ __launch_bounds__(BLOCK) void k_step_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                    const uint8_t* __restrict__ kind, const uint8_t* __restrict__ wdist,
+                                                    const MlpDev* __restrict__ net, KParams P, StepArgs A) {
+  __shared__ short feats[BLOCK / 64][128 * MLP_IN];
+  __shared__ float nus[BLOCK / 64][128];
+  const uint32_t b = blockIdx.x;
+  if (b < A.nwb) {
+    wall_cell<D3Q19, S>(fin, fout, A, P, (int64_t)b * BLOCK + threadIdx.x);
+    return;
+  }
+  const int w = threadIdx.x >> 6;
+  bulk_block<D3Q19, S, CPT, REG, true>(fin, fout, kind, P, A, b - A.nwb, wdist, net, feats[w], nus[w]);
 }
 
 // One lattice step of many independent domains in one launch (sweeps): the same wall and bulk
@@ -833,6 +973,27 @@ void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs
   const int cpt = cells_per_thread(c.P);
   const bool reg = c.P.collision != 1;
   const dim3 grid(A.nwb + A.nbb);
+  if (c.P.closure == 1 && c.mlp && c.wdist && c.P.q == 19) {
+    auto go = [&](auto kern, auto s) {
+      using S = decltype(s);
+      hipLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind, c.wdist, c.mlp,
+                         c.P, A);
+    };
+    auto pick = [&](auto s) {
+      using S = decltype(s);
+      if (cpt == 2) {
+        if (reg) go(k_step_mlp<S, 2, true>, s);
+        else go(k_step_mlp<S, 2, false>, s);
+      } else {
+        if (reg) go(k_step_mlp<S, 1, true>, s);
+        else go(k_step_mlp<S, 1, false>, s);
+      }
+    };
+    if (c.prec == 0) pick(SF32{});
+    else if (c.prec == 1) pick(SF16{});
+    else pick(SBF16{});
+    return;
+  }
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
     auto go = [&](auto kern) {

@@ -1,6 +1,7 @@
 // Host-side launch interface between the C-ABI layer and the kernels.
 #pragma once
 #include "lbm.hpp"
+#include "mlp.hpp"
 
 namespace wt {
 
@@ -12,6 +13,8 @@ struct LaunchCtx {
   const uint32_t* sdf;
   uint8_t* kind;
   const KWheels* W;
+  const uint8_t* wdist;     // wall distance bytes (learned closure), may be null
+  const MlpDev* mlp;        // learned closure network (device), null: Smagorinsky
 };
 
 void launch_init(const LaunchCtx& c, void* f);
@@ -33,5 +36,20 @@ void launch_unpack(const LaunchCtx& c, const void* f, float* out);
 void launch_pack(const LaunchCtx& c, const float* in, void* f);
 void launch_copy(hipStream_t st, const void* a, void* b, int64_t bytes);
 void launch_clear_stats(const LaunchCtx& c, void* meanA, float* meanB);
+
+// ---- learned closure (kernels_mlp.hip) ----
+void launch_wall_distance(hipStream_t st, const uint32_t* flags, uint32_t* tmp, uint8_t* out, int nx, int ny, int nz);
+int64_t coarse_blocks(int64_t nc);
+void launch_coarse_fields(hipStream_t st, int prec, const void* f, const uint32_t* flags, int fnx, int fny, int fnz, float* cf,
+                          uint8_t* bad);
+void launch_coarse_count(hipStream_t st, const float* cf, const uint8_t* bad, const uint8_t* wd, int cnx, int cny, int cnz, float nu0,
+                         uint32_t* counts);
+void launch_coarse_samples(hipStream_t st, const float* cf, const uint8_t* bad, const uint8_t* wd, int cnx, int cny, int cnz, float nu0,
+                           const uint64_t* offsets, float* out);
+int64_t grad_partial_floats(int64_t samples);
+void launch_mlp_grad(hipStream_t st, const float* data, const uint32_t* perm, int64_t first, int64_t count, float inv_batch,
+                     const float* params, const MlpNorm& norm, float* partials, float* grad, double* lossacc);
+void launch_adam(hipStream_t st, float* p, float* m, float* v, const float* g, float lr, float b1, float b2, float eps, int64_t t);
+void launch_mlp_infer(hipStream_t st, const MlpDev* net, const float* raw, int64_t n, float* nu);
 
 }  // namespace wt

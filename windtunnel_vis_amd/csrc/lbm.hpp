@@ -309,6 +309,8 @@ struct Relax {
   float k1;            // 4.5 (1 - omega) (regularized)
   float omega, om1;    // 1 / tau, 1 - 1 / tau
   float sIn;           // inlet absorbing-layer weight (0 outside the layer)
+  float t0;            // tau before the closure (tau0 + outlet sponge, ground-wall floor)
+  float E0, trP;       // E = E0 - k1 trP (regularized) or E0 (BGK)
 };
 
 template <class L>
@@ -377,15 +379,25 @@ __device__ __forceinline__ Relax relax(const float (&d)[L::Q], float xpos, bool 
   m.pxx = pxx; m.pyy = pyy; m.pzz = pzz; m.pxy = pxy; m.pxz = pxz; m.pyz = pyz;
   m.omega = omega;
   m.om1 = om1;
+  m.t0 = t0;
+  m.E0 = drho - 1.5f * rho * usq;
+  m.trP = D3 ? (pxx + pyy + pzz) * third : (pxx + pyy) * third;
   if (reg) {
     m.k1 = 4.5f * om1;
-    const float trP = D3 ? (pxx + pyy + pzz) * third : (pxx + pyy) * third;
-    m.E = drho - 1.5f * rho * usq - m.k1 * trP;
+    m.E = m.E0 - m.k1 * m.trP;
   } else {
     m.k1 = 0.0f;
-    m.E = drho - 1.5f * rho * usq;
+    m.E = m.E0;
   }
   return m;
+}
+
+// Replaces the relaxation time of a Relax record (the learned closure's tau).
+__device__ __forceinline__ void retau(Relax& m, float tau, bool reg) {
+  m.omega = fast_rcp(tau);
+  m.om1 = 1.0f - m.omega;
+  m.k1 = reg ? 4.5f * m.om1 : 0.0f;
+  m.E = reg ? m.E0 - m.k1 * m.trP : m.E0;
 }
 
 // Inlet absorbing layer: mix toward feq(1, U), wgslLbm.ts:220, :238.

@@ -1,0 +1,129 @@
+// Learned subgrid closure: a per-cell MLP 8-32-32-1 (ReLU hidden layers, softplus output) that
+// predicts the eddy viscosity nu_t in place of Smagorinsky (the collision's tau, wgslLbm.ts:214).
+// Shared by the step kernel (BF16 MFMA inference), the training kernels (FP32) and the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+
+namespace wt {
+
+constexpr int MLP_IN = 8;
+constexpr int MLP_H = 32;
+// FP32 master parameters, in this order: W1[32][8] (out x in), b1[32], W2[32][32], b2[32], w3[32], b3
+constexpr int MLP_OFF_W1 = 0;
+constexpr int MLP_OFF_B1 = 256;
+constexpr int MLP_OFF_W2 = 288;
+constexpr int MLP_OFF_B2 = 1312;
+constexpr int MLP_OFF_W3 = 1344;
+constexpr int MLP_OFF_B3 = 1376;
+constexpr int MLP_NPARAM = 1377;
+
+// Feature scaling: invariants of degree k are compressed as sign(x) log(1 + |x| / S_REF^k).
+constexpr float MLP_S_REF = 1e-3f;
+
+// Input normalisation and output scale that travel with the weights.
+struct MlpNorm {
+  float mu[MLP_IN];
+  float isig[MLP_IN];  // 1 / sigma
+  float nu_scale;      // nu_t = nu_scale * softplus(z)
+  float pad[3];
+};
+
+__host__ __device__ inline float slog(float x, float ref) {
+  const float a = log1pf(fabsf(x) / ref);
+  return x < 0.0f ? -a : a;
+}
+
+// The eight raw (unnormalised) inputs of a cell: six rotation invariants of the strain rate S
+// (I1 = tr S, I2 = (I1^2 - tr S^2) / 2, I3 = det S, tr S^2, tr S^3, |S| = sqrt(2 tr S^2)), the wall
+// distance d (cells, capped at 4) and the local Reynolds number |u| max(d, 1/2) / nu0.
+__host__ __device__ inline void mlp_raw_features(float sxx, float syy, float szz, float sxy, float sxz, float syz, float wd,
+                                                 float umag, float nu0, float (&x)[MLP_IN]) {
+  const float i1 = sxx + syy + szz;
+  const float tr2 = sxx * sxx + syy * syy + szz * szz + 2.0f * (sxy * sxy + sxz * sxz + syz * syz);
+  const float i2 = 0.5f * (i1 * i1 - tr2);
+  const float i3 = sxx * (syy * szz - syz * syz) - sxy * (sxy * szz - syz * sxz) + sxz * (sxy * syz - syy * sxz);
+  const float tr3 = i1 * i1 * i1 - 3.0f * i1 * i2 + 3.0f * i3;  // Cayley-Hamilton
+  const float m = sqrtf(2.0f * tr2);
+  const float r = MLP_S_REF, r2 = r * r, r3 = r2 * r;
+  x[0] = slog(i1, r);
+  x[1] = slog(i2, r2);
+  x[2] = slog(i3, r3);
+  x[3] = slog(tr2, r2);
+  x[4] = slog(tr3, r3);
+  x[5] = slog(m, r);
+  x[6] = wd;
+  x[7] = log1pf(umag * fmaxf(wd, 0.5f) / nu0);
+}
+
+__host__ __device__ inline float softplus(float z) { return z > 20.0f ? z : log1pf(expf(z)); }
+
+// ---------------------------------------------------------------------------------------------
+// Wave-wide BF16 MFMA inference (v_mfma_f32_32x32x8bf16_1k). A wave evaluates 128 cells in four
+// groups of 32, in transposed form so that no activation is moved between lanes:
+//   H1^T (32 x 32 cells) = W1 (32 x 8) X^T (8 x 32 cells) + b1,   relu
+//   H2^T (32 x 32 cells) = W2 (32 x 32) H1^T + b2 (four K = 8 chunks),   relu
+//   z = w3 . h2 + b3 (16 hidden units per lane, the other 16 in lane l ^ 32), nu_t = s softplus(z).
+// Operand layouts (checked on gfx950): A (32 x 8): lane l holds row l % 32, k = 4 (l / 32) + 0..3;
+// B (8 x 32): lane l holds column l % 32, the same k; D (32 x 32): lane l, register r holds row
+// 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32. Register block c of D (r = 4c..4c+3) is therefore
+// exactly the B operand of K-chunk c of the next layer.
+// ---------------------------------------------------------------------------------------------
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__device__ inline short to_bf16(float x) {  // round to nearest even (finite inputs)
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// Device-side network in the layout the MFMA path reads (prepared once from the FP32 master).
+struct MlpDev {
+  bf16x4_t w1a[64];     // A operand of layer 1 per lane
+  bf16x4_t w2a[4][64];  // A operands of layer 2 per K-chunk and lane
+  float b1d[2][16];     // layer-1 bias in D layout, per lane half
+  float b2d[2][16];     // layer-2 bias in D layout
+  float w3d[2][16];     // output weights in D layout
+  float b3;
+  float pad[3];
+  MlpNorm norm;
+};
+
+// Evaluates nu_t for the 128 cells of a wave. feats: this wave's LDS staging area of 128 x 8 BF16
+// (cell-major; every lane wrote its own cells before the call), nu: 128 floats of LDS for the
+// results. All 64 lanes must call it (MFMA uses the whole wave); cells are indexed 0..127.
+__device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* feats, float* nu) {
+  const int l = threadIdx.x & 63;
+  const int h = l >> 5;
+  const bf16x4_t w1 = net->w1a[l];
+  bf16x4_t w2[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) w2[c] = net->w2a[c][l];
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const int cell = 32 * g + (l & 31);
+    const bf16x4_t xb = *(const bf16x4_t*)(feats + cell * MLP_IN + 4 * h);
+    f32x16_t acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = net->b1d[h][r];
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1, xb, acc, 0, 0, 0);
+    bf16x4_t hb[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int t = 0; t < 4; t++) hb[c][t] = to_bf16(fmaxf(acc[4 * c + t], 0.0f));
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = net->b2d[h][r];
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2[c], hb[c], acc, 0, 0, 0);
+    float z = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) z = fmaf(net->w3d[h][r], fmaxf(acc[r], 0.0f), z);
+    z += __shfl_xor(z, 32);
+    if (h == 0) nu[cell] = net->norm.nu_scale * softplus(z + net->b3);
+  }
+}
+
+}  // namespace wt

@@ -1,0 +1,92 @@
+// Private to libwt: the solver handle's state and the helpers the entry-point files share.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+
+#include "../../include/wt.h"
+#include "launch.hpp"
+#include "mlp.hpp"
+
+namespace wt {
+// records the message for wt_last_error and returns the code
+int fail(int code, const std::string& msg);
+}  // namespace wt
+
+#define HIP_TRY(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return wt::fail(WT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct wt_solver {
+  // (the opaque handle of include/wt.h)
+  int device = 0;
+  hipStream_t stream = nullptr;
+  wt_dims dims{};
+  int64_t n = 0;
+  int q = 0;
+  int prec = 0;
+  wt_params prm{};
+  wt::KParams P{};
+  wt::KWheels W{};
+  void* f[2] = {nullptr, nullptr};
+  uint32_t* flags = nullptr;
+  uint32_t* sdf = nullptr;
+  uint8_t* kind = nullptr;
+  uint32_t* wlist = nullptr;   // cells handled by the wall kernel
+  uint32_t* wmask = nullptr;   // their solid masks
+  wt::WallLink* wlink = nullptr;   // their links, (q - 1) x wcap
+  uint32_t* wmeta = nullptr;   // their meta words
+  int64_t wcap = 0;
+  unsigned long long* counts = nullptr;
+  ushort4* vel = nullptr;
+  float4* meanA = nullptr;
+  float* meanB = nullptr;
+  double* partials = nullptr;
+  double* result = nullptr;
+  double* h_result = nullptr;  // pinned
+  int64_t nwall = 0, nfast = 0, nsolid = 0;  // cells: wall kernel, bulk fluid, bulk solid
+  int parity = 0;
+  bool pending_init = true;
+  int64_t step = 0;
+  int64_t bytes = 0;
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev;   // per step: start, stop of the step kernel
+  int item_chunk = 0;           // WT_TUNE_ITEM_CHUNK (0: one contiguous eighth per XCD)
+  int cpt = 2;                  // WT_TUNE_CELLS_PER_THREAD (2 needs an even nx)
+  size_t ev_used = 0;
+  double acc_bulk = 0, acc_bnd = 0;
+  int64_t acc_launches = 0;
+  // learned closure
+  uint8_t* wdist = nullptr;     // wall distance per cell (cells x 32, capped at 4 cells); built on demand
+  bool wdist_valid = false;
+  wt::MlpDev* mlp = nullptr;        // network in the MFMA layout (device)
+
+  wt::LaunchCtx ctx() const {
+    wt::LaunchCtx c;
+    c.stream = stream;
+    c.P = P;
+    c.prec = prec;
+    c.flags = flags;
+    c.sdf = sdf;
+    c.kind = kind;
+    c.W = &W;
+    c.wdist = wdist;
+    c.mlp = mlp;
+    return c;
+  }
+};
+
+namespace wt {
+void derive(wt_solver* s);
+int run_init(wt_solver* s);
+int prep_walls(wt_solver* s);
+int reclassify(wt_solver* s);
+int do_steps(wt_solver* s, int steps);
+int do_sample(wt_solver* s);
+void decode_stats(const wt_solver* s, wt_stats* out);
+int64_t storage_bytes(const wt_solver* s);  // bytes of one population buffer
+int ensure_wdist(wt_solver* s);             // wall distances for the learned closure
+}  // namespace wt

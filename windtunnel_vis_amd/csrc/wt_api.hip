@@ -9,83 +9,21 @@
 #include <string>
 #include <vector>
 
-#include "../../include/wt.h"
-#include "launch.hpp"
+#include "solver.hpp"
 #include "voxel.hpp"
 
 using namespace wt;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int wt::fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
-#define HIP_TRY(call)                                                                          \
-  do {                                                                                         \
-    hipError_t e_ = (call);                                                                    \
-    if (e_ != hipSuccess) return fail(WT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-}  // namespace
-
-struct wt_solver {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  wt_dims dims{};
-  int64_t n = 0;
-  int q = 0;
-  int prec = 0;
-  wt_params prm{};
-  KParams P{};
-  KWheels W{};
-  void* f[2] = {nullptr, nullptr};
-  uint32_t* flags = nullptr;
-  uint32_t* sdf = nullptr;
-  uint8_t* kind = nullptr;
-  uint32_t* wlist = nullptr;   // cells handled by the wall kernel
-  uint32_t* wmask = nullptr;   // their solid masks
-  WallLink* wlink = nullptr;   // their links, (q - 1) x wcap
-  uint32_t* wmeta = nullptr;   // their meta words
-  int64_t wcap = 0;
-  unsigned long long* counts = nullptr;
-  ushort4* vel = nullptr;
-  float4* meanA = nullptr;
-  float* meanB = nullptr;
-  double* partials = nullptr;
-  double* result = nullptr;
-  double* h_result = nullptr;  // pinned
-  int64_t nwall = 0, nfast = 0, nsolid = 0;  // cells: wall kernel, bulk fluid, bulk solid
-  int parity = 0;
-  bool pending_init = true;
-  int64_t step = 0;
-  int64_t bytes = 0;
-  // profiling
-  bool prof = false;
-  std::vector<hipEvent_t> ev;   // per step: start, stop of the step kernel
-  int item_chunk = 0;           // WT_TUNE_ITEM_CHUNK (0: one contiguous eighth per XCD)
-  int cpt = 2;                  // WT_TUNE_CELLS_PER_THREAD (2 needs an even nx)
-  size_t ev_used = 0;
-  double acc_bulk = 0, acc_bnd = 0;
-  int64_t acc_launches = 0;
-
-  LaunchCtx ctx() const {
-    LaunchCtx c;
-    c.stream = stream;
-    c.P = P;
-    c.prec = prec;
-    c.flags = flags;
-    c.sdf = sdf;
-    c.kind = kind;
-    c.W = &W;
-    return c;
-  }
-};
-
-namespace {
+namespace wt {
 
 // SolverGPU.writeParams, SolverGPU.ts:160-187.
 void derive(wt_solver* s) {
@@ -150,6 +88,7 @@ int prep_walls(wt_solver* s) {
 }
 
 int reclassify(wt_solver* s) {
+  s->wdist_valid = false;  // flags changed
   HIP_TRY(hipMemsetAsync(s->counts, 0, 4 * sizeof(unsigned long long), s->stream));
   LaunchCtx c = s->ctx();
   launch_classify(c, s->wlist, s->wmask, s->counts);
@@ -179,6 +118,9 @@ int do_steps(wt_solver* s, int steps) {
   if (s->pending_init) {
     int r = run_init(s);
     if (r) return r;
+  }
+  if (s->P.closure == 1 && s->mlp && !s->wdist_valid) {
+    if (int r = ensure_wdist(s)) return r;
   }
   LaunchCtx c = s->ctx();
   const StepArgs A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->item_chunk);
@@ -230,7 +172,9 @@ void decode_stats(const wt_solver* s, wt_stats* out) {
   out->step = s->step;
 }
 
-}  // namespace
+int64_t storage_bytes(const wt_solver* s) { return (int64_t)s->q * s->n * (s->prec == WT_PREC_F32 ? 4 : 2); }
+
+}  // namespace wt
 
 // A batch of independent domains stepped by one launch per step (k_step_batch).
 struct wt_batch {
@@ -388,7 +332,7 @@ int wt_destroy(wt_solver* s) {
   if (!s) return WT_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto e : s->ev) (void)hipEventDestroy(e);
-  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
+  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->wdist, s->mlp, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->h_result) (void)hipHostFree(s->h_result);
