@@ -185,6 +185,26 @@ def sweep_bench(steps=500, total=2000, chunk=250):
             "cd_steps": total, "cases": per_case}
 
 
+def closure_bench():
+    """Learned vs plain time-to-C_D: the truck + trailer at 512x256x256 (plain Smagorinsky, the
+    reference C_D and the training data) and at 256x128x128 with plain Smagorinsky and with the
+    trained network (windtunnel_vis_amd.closure_study)."""
+    from windtunnel_vis_amd import closure_study
+    r = closure_study.study(fine_steps=24000, coarse_steps=24000, snapshot_from=10000, snapshot_every=2000, epochs=6)
+
+    def run(x):
+        return {"cd": round(x["cd_mean"], 4) if x["cd_mean"] == x["cd_mean"] else None, "converged": x["converged"],
+                "converged_s": None if x["converged_wall"] is None else round(x["converged_wall"], 3),
+                "within_2pct_of_fine_s": None if x["within2_wall"] is None else round(x["within2_wall"], 3),
+                "mlups": round(x["mlups"], 1), "unstable": x["unstable"]}
+
+    return {"body": "truck + trailer", "fine": run(r["fine"]), "coarse_plain": run(r["coarse_plain"]),
+            "coarse_learned": run(r["coarse_learned"]), "cd_err_plain": round(r["cd_err_plain"], 4),
+            "cd_err_learned": round(r["cd_err_learned"], 4), "training_records": r["records"],
+            "train_seconds": round(r["train_seconds"], 2), "train_loss": [round(x, 5) for x in r["train_loss"]],
+            "smag_floor": r["network"].smag_floor}
+
+
 def cpu_baseline(dims, sample_steps, threads):
     """The CPU restatement (FP32 arithmetic path, OpenMP) on the same domain for a few steps."""
     from oracle.oracle import OracleSolver
@@ -218,6 +238,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU-baseline sample steps (0 = auto)")
     ap.add_argument("--north-star", action="store_true", help="also time 512x256x256")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched 24-domain sweep")
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure study")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: copy probe + the main workload only")
     ap.add_argument("--chunk", type=int, default=None,
@@ -228,7 +249,7 @@ def main():
         sys.exit(2)
     dims = tuple(int(v) for v in args.dims.split(","))
     if args.profile_run:
-        args.no_physics = args.no_cpu = args.no_sweep = True
+        args.no_physics = args.no_cpu = args.no_sweep = args.no_closure = True
         args.north_star = False
     copy_gbs = copy_probe(256 << 20, 3) if args.profile_run else copy_probe()
     r = run_gpu(dims, args.steps, args.warmup, args.precision, chunk=args.chunk)
@@ -278,6 +299,8 @@ def main():
                                    "unstable": c1["unstable"], "seconds": round(c1["seconds"], 2)}
     if not args.no_sweep:
         out["batched_sweep"] = sweep_bench()
+    if not args.no_closure:
+        out["learned_closure"] = closure_bench()
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         steps = args.cpu_steps or 100

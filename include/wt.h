@@ -276,14 +276,16 @@ int wt_batch_stats(wt_batch* b, wt_stats* out);
  * and third principal invariants, tr S^2, tr S^3, |S|; each compressed as sign(x) log(1 + |x| /
  * 1e-3^degree)), the wall distance (cells to the nearest solid cell centre, capped at 4) and the
  * local Reynolds number |u| max(d, 1/2) / nu. Inference runs in the step kernel on BF16 MFMA
- * (FP32 accumulation) for bulk cells; cells next to the body and non-finite outputs keep
+ * (FP32 accumulation) for bulk cells. Smagorinsky guards it: nu_t is at least smag_floor times
+ * the Smagorinsky eddy viscosity of the cell, cells next to the body and non-finite outputs keep
  * Smagorinsky, which is also the closure when no network is loaded. D3Q19 only. */
 #define WT_MLP_NPARAM 1377
 typedef struct wt_mlp {
   float params[WT_MLP_NPARAM]; /* W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3 (out x in, row-major) */
   float mu[8];                 /* input normalisation: (x - mu) / sigma */
   float sigma[8];
-  float nu_scale;              /* nu_t = nu_scale * softplus(z) */
+  float nu_scale;              /* nu_t = nu_scale * min(softplus(z), 10) */
+  float smag_floor;            /* nu_t >= smag_floor * Smagorinsky's nu_t */
 } wt_mlp;
 int wt_mlp_load(wt_solver* s, const wt_mlp* net);
 /* Wall distance per cell as bytes (32 per cell, capped at 128). */
@@ -295,7 +297,8 @@ int wt_mlp_infer(const wt_mlp* net, const float* raw, int64_t n, float* nu);
  * device memory. wt_dataset_add box-filters a fine D3Q19 run 2x onto the grid of `coarse` (same
  * geometry at half resolution; it supplies the wall distances and nu): per coarse cell the block
  * mean velocity, the subgrid stress tau_ij = <u_i u_j> - <u_i><u_j>, the filtered strain rate by
- * central differences and the least-squares eddy viscosity nu_t = max(0, -tau_dev:S / (2 S:S));
+ * central differences and the least-squares eddy viscosity of the 3x3x3 neighbourhood,
+ * nu_t = max(0, sum(-tau_dev:S) / sum(2 S:S));
  * cells touching a solid or a domain face are skipped. Records are appended in cell order. */
 typedef struct wt_dataset wt_dataset;
 int wt_dataset_create(int64_t capacity, wt_dataset** out);
@@ -306,7 +309,7 @@ int wt_dataset_read(const wt_dataset* d, int64_t first, int64_t count, float* ou
 int wt_dataset_append(wt_dataset* d, const float* records, int64_t count);
 
 /* GPU training with hand-written kernels (FP32 forward / backward, FP64 gradient reduction, Adam):
- * loss = mean (softplus(z) - min(nu_t / nu_scale, 100))^2 over mini-batches of `batch` records in
+ * loss = mean (softplus(z) - min(nu_t / nu_scale, 10))^2 over mini-batches of `batch` records in
  * a fresh random order every epoch (`max_samples` > 0 limits the records drawn per epoch).
  * net->params is the initial state on entry and the trained state on return; mu, sigma and
  * nu_scale are used as given. epoch_loss (epochs entries) may be NULL. */

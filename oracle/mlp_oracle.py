@@ -62,11 +62,15 @@ def forward(p, xn, nu_scale):
     a2 = h1 @ W2.T + b2
     h2 = np.maximum(a2, 0)
     z = h2 @ w3 + b3
-    return nu_scale * softplus(z), (xn, a1, h1, a2, h2, z)
+    return nu_scale * np.minimum(softplus(z), TARGET_CLIP), (xn, a1, h1, a2, h2, z)
+
+
+TARGET_CLIP = 10.0
 
 
 def loss_and_grad(p, xn, target):
-    """Mean squared error of softplus(z) against target (nu_t / nu_scale); gradient w.r.t. p."""
+    """Mean squared error of softplus(z) against target (nu_t / nu_scale, clipped at TARGET_CLIP
+    by the caller); gradient w.r.t. p."""
     W1, b1, W2, b2, w3, b3 = unpack(p)
     _, (x, a1, h1, a2, h2, z) = forward(p, xn, 1.0)
     pred = softplus(z)
@@ -100,7 +104,7 @@ def forward_bf16(p, xn, nu_scale):
     h1 = np.maximum(bf16(xn) @ bf16(W1).T + b1, 0)
     h2 = np.maximum(bf16(h1) @ bf16(W2).T + b2, 0)
     z = h2 @ w3 + b3
-    return nu_scale * softplus(z)
+    return nu_scale * np.minimum(softplus(z), TARGET_CLIP)
 
 
 def adam(p, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
@@ -135,8 +139,9 @@ def wall_distance(flags, dims, radius=4):
 def coarse_samples(u_fine, solid_fine, dims_fine, wd_coarse, nu0):
     """Training samples from a fine velocity field (n_f, 3): 2x box filter, subgrid stress
     tau = <u u> - <u><u>, filtered strain by central differences on the coarse grid, target
-    nu_t = max(0, -tau_dev : S / (2 S : S)). A coarse cell is used when its 8 fine cells and the 8
-    fine cells of each of its 6 neighbours are fluid. Returns (features (m, 8), target (m,), index (m,))."""
+    nu_t = max(0, sum(-tau_dev : S) / sum(2 S : S)) over the 3x3x3 neighbourhood (cells without a
+    strain contribute 0). A coarse cell is used when its 8 fine cells and the 8 fine cells of each
+    of its 6 neighbours are fluid. Returns (features (m, 8), target (m,), index (m,))."""
     nx, ny, nz = dims_fine
     cx, cy, cz = nx // 2, ny // 2, nz // 2
     u = np.asarray(u_fine, dtype=np.float64).reshape(nz, ny, nx, 3)
@@ -160,9 +165,17 @@ def coarse_samples(u_fine, solid_fine, dims_fine, wd_coarse, nu0):
     S = 0.5 * (g + np.swapaxes(g, -1, -2))
     tr = np.trace(tau, axis1=-2, axis2=-1)
     tdev = tau - tr[..., None, None] / 3.0 * np.eye(3)
-    ss = np.einsum("...ij,...ij->...", S, S)
-    num = -np.einsum("...ij,...ij->...", tdev, S)
-    nut = np.maximum(0.0, num / (2.0 * ss + 1e-30))
+    ss = np.where(valid, 2.0 * np.einsum("...ij,...ij->...", S, S), 0.0)
+    num = np.where(valid, -np.einsum("...ij,...ij->...", tdev, S), 0.0)
+    # least-squares eddy viscosity of the 3x3x3 neighbourhood (valid cells are interior, so no wrap)
+    N3 = np.zeros_like(num)
+    D3 = np.zeros_like(ss)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                N3 += np.roll(num, (-dz, -dy, -dx), axis=(0, 1, 2))
+                D3 += np.roll(ss, (-dz, -dy, -dx), axis=(0, 1, 2))
+    nut = np.maximum(0.0, N3 / (D3 + 1e-30))
     idx = np.flatnonzero(valid.reshape(-1))
     Sv = S.reshape(-1, 3, 3)[idx]
     S6 = np.stack([Sv[:, 0, 0], Sv[:, 1, 1], Sv[:, 2, 2], Sv[:, 0, 1], Sv[:, 0, 2], Sv[:, 1, 2]], axis=1)

@@ -97,7 +97,7 @@ def test_gradient_matches_restatement():
     loss, g = closure.gradient(ds, net, first, count)
     r = rec[first:first + count].astype(np.float64)
     xn = (r[:, :8] - net.mu) / net.sigma
-    lref, gref = M.loss_and_grad(net.params.astype(np.float64), xn, np.minimum(r[:, 8] / net.nu_scale, 100))
+    lref, gref = M.loss_and_grad(net.params.astype(np.float64), xn, np.minimum(r[:, 8] / net.nu_scale, M.TARGET_CLIP))
     assert abs(loss - lref) < 1e-5 * lref
     assert np.max(np.abs(g - gref)) < 1e-4 * np.max(np.abs(gref))
     ds.destroy()
@@ -113,7 +113,7 @@ def test_full_batch_adam_steps_match_restatement():
     losses = closure.train(ds, net, epochs=3, batch=512, lr=1e-3)
     r = rec.astype(np.float64)
     xn = (r[:, :8] - net.mu) / net.sigma
-    t = np.minimum(r[:, 8] / net.nu_scale, 100)
+    t = np.minimum(r[:, 8] / net.nu_scale, M.TARGET_CLIP)
     p, m, v = p0, np.zeros_like(p0), np.zeros_like(p0)
     ref_losses = []
     for k in range(1, 4):
@@ -155,24 +155,49 @@ def test_mfma_inference_matches_bf16_restatement():
 
 def test_constant_network_equals_plain_collision_with_added_viscosity():
     # a network that outputs nu_t = c everywhere turns the MLP closure into the plain collision with
-    # nu + c (no body: no wall cells; free-slip ground: no ground-wall floor)
+    # nu + c (no body: no wall cells; free-slip ground: no ground-wall floor; Cs = 0: the
+    # Smagorinsky guard is 0)
     from windtunnel_vis_amd import SolverGPU, closure
     dims = (32, 16, 12)
     _, params, _, _, _ = make_case("d3_full")
-    p = dict(params, ground="freeslip", precision="f32", cs=0.1, nu=0.01)
+    p = dict(params, ground="freeslip", precision="f32", cs=0.0, nu=0.01)
     flags = np.zeros(32 * 16 * 12, np.uint32)
     sdf = np.full(32 * 16 * 12, -1.0, np.float32)
     c = 2e-3
-    net = closure.Network(nu_scale=1e-3)
+    net = closure.Network(nu_scale=1e-3, smag_floor=0.0)
     net.params[M.OFF_B3] = np.log(np.expm1(c / 1e-3))  # softplus(b3) * 1e-3 = c
     a = _solver(dims, p, flags, sdf, [])
     closure.attach(a, net)
-    b = _solver(dims, dict(p, cs=0.0, nu=0.01 + c), flags, sdf, [])
+    b = _solver(dims, dict(p, nu=0.01 + c), flags, sdf, [])
     a.step(20)
     b.step(20)
     fa, fb = a.readPopulations(), b.readPopulations()
     err = np.max(np.abs(fa - fb)) / np.max(np.abs(fb))
     print("constant-network vs plain: %.2e" % err, flush=True)
     assert err < 2e-5
+    a.destroy()
+    b.destroy()
+
+
+def test_zero_network_with_full_guard_is_smagorinsky():
+    # with smag_floor = 1 and a network that outputs ~0 the closure is Smagorinsky everywhere: the
+    # whole learned path (features, MFMA, retau, ground-wall BGK rows, skipped and solid lanes) on
+    # a geometry with a body, wheels and a moving belt reproduces the plain run
+    from windtunnel_vis_amd import closure
+    dims = (40, 16, 16)
+    flags, sdf, wheels = box_body_3d(dims)
+    _, params, _, _, _ = make_case("d3_full")
+    p = dict(params, precision="f32")
+    net = closure.Network(nu_scale=1e-3, smag_floor=1.0)
+    net.params[M.OFF_B3] = -30.0
+    a = _solver(dims, p, flags, sdf, wheels)
+    closure.attach(a, net)
+    b = _solver(dims, p, flags, sdf, wheels)
+    a.step(50)
+    b.step(50)
+    fa, fb = a.readPopulations(), b.readPopulations()
+    err = np.max(np.abs(fa - fb)) / np.max(np.abs(fb))
+    print("zero network + guard vs Smagorinsky: %.2e" % err, flush=True)
+    assert err < 1e-5
     a.destroy()
     b.destroy()

@@ -76,7 +76,8 @@ MLP_NPARAM = 1377
 
 
 class Mlp(C.Structure):
-    _fields_ = [("params", C.c_float * MLP_NPARAM), ("mu", C.c_float * 8), ("sigma", C.c_float * 8), ("nu_scale", C.c_float)]
+    _fields_ = [("params", C.c_float * MLP_NPARAM), ("mu", C.c_float * 8), ("sigma", C.c_float * 8), ("nu_scale", C.c_float),
+                ("smag_floor", C.c_float)]
 
 
 class TrainOpts(C.Structure):

@@ -14,21 +14,22 @@ NPARAM = N.MLP_NPARAM
 class Network:
     """FP32 master parameters (W1[32][8], b1, W2[32][32], b2, w3, b3) + input normalisation."""
 
-    def __init__(self, params=None, mu=None, sigma=None, nu_scale=1e-3):
+    def __init__(self, params=None, mu=None, sigma=None, nu_scale=1e-3, smag_floor=1.0):
         self.params = np.zeros(NPARAM, np.float32) if params is None else np.asarray(params, np.float32).copy()
         self.mu = np.zeros(8, np.float32) if mu is None else np.asarray(mu, np.float32).copy()
         self.sigma = np.ones(8, np.float32) if sigma is None else np.asarray(sigma, np.float32).copy()
         self.nu_scale = float(nu_scale)
+        self.smag_floor = float(smag_floor)
 
     @classmethod
-    def initial(cls, seed=20261203, nu_scale=1e-3):
+    def initial(cls, seed=20261203, nu_scale=1e-3, smag_floor=1.0):
         """He-normal hidden layers, small output layer, zero biases."""
         rng = np.random.default_rng(seed)
         p = np.zeros(NPARAM, np.float32)
         p[0:256] = rng.normal(0, np.sqrt(2 / 8), 256)
         p[288:1312] = rng.normal(0, np.sqrt(2 / 32), 1024)
         p[1344:1376] = rng.normal(0, np.sqrt(1 / 32), 32)
-        return cls(p, nu_scale=nu_scale)
+        return cls(p, nu_scale=nu_scale, smag_floor=smag_floor)
 
     def to_struct(self):
         m = N.Mlp()
@@ -37,15 +38,17 @@ class Network:
             m.mu[i] = float(self.mu[i])
             m.sigma[i] = float(self.sigma[i])
         m.nu_scale = self.nu_scale
+        m.smag_floor = self.smag_floor
         return m
 
     def save(self, path):
-        np.savez(path, params=self.params, mu=self.mu, sigma=self.sigma, nu_scale=np.float64(self.nu_scale))
+        np.savez(path, params=self.params, mu=self.mu, sigma=self.sigma, nu_scale=np.float64(self.nu_scale),
+                 smag_floor=np.float64(self.smag_floor))
 
     @classmethod
     def load(cls, path):
         z = np.load(path)
-        return cls(z["params"], z["mu"], z["sigma"], float(z["nu_scale"]))
+        return cls(z["params"], z["mu"], z["sigma"], float(z["nu_scale"]), float(z["smag_floor"]))
 
 
 class Dataset:

@@ -451,8 +451,8 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
 // Bulk finish with the learned closure (D3Q19): the moments of both cells are formed with the
 // Smagorinsky tau as a first estimate, the strain rate S = -3 Pi^neq / (2 rho tau) and the other
 // inputs of every cell of the wave go through the network on the MFMA path (all 64 lanes take
-// part, so the skip tests come after it), and each cell relaxes with tau = t0 + 3 nu_t; a
-// non-finite nu_t keeps Smagorinsky.
+// part, so the skip tests come after it), and each cell relaxes with tau = t0 + 3 nu_t,
+// nu_t = max(network, smag_floor x Smagorinsky); a non-finite output keeps Smagorinsky.
 template <class L, class S, int CPT, bool REG>
 __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout, const KParams& P, const RowPos& r,
                                                 const BulkIn<L, S, CPT>& in, const MlpDev* __restrict__ net, short* feats,
@@ -535,11 +535,16 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
   mlp_wave_nu(net, feats, nus);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const float nua = nus[2 * lane], nub = nus[2 * lane + 1];
-  if (nua == nua && nua < 1e3f) retau(ma, ma.t0 + 3.0f * nua, reg);
-  if constexpr (CPT == 2) {
-    if (nub == nub && nub < 1e3f) retau(mb, mb.t0 + 3.0f * nub, reg);
-  }
+  // nu_t = max(network, floor x nu_smag), nu_smag = (tau_smag - t0) / 3; non-finite: Smagorinsky
+  const float floor_ = net->norm.smag_floor;
+  auto closure_tau = [&](Relax& m, float nu) {
+    if (nu == nu && nu < 1e3f) {
+      const float nus_ = (fast_rcp(m.omega) - m.t0) * (1.0f / 3.0f);
+      retau(m, m.t0 + 3.0f * fmaxf(nu, floor_ * nus_), reg);
+    }
+  };
+  closure_tau(ma, nus[2 * lane]);
+  if constexpr (CPT == 2) closure_tau(mb, nus[2 * lane + 1]);
   if (x0 >= NX) return;
   if (k == SOLID_ALL) {
 #pragma unroll

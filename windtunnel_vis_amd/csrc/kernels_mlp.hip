@@ -80,9 +80,10 @@ void launch_wall_distance(hipStream_t st, const uint32_t* flags, uint32_t* tmp, 
 // Training data from a fine run. Pass 1 (per coarse cell = 2x2x2 fine cells): the block mean
 // velocity and the subgrid stress tau_ij = <(u_i - <u_i>)(u_j - <u_j>)> (two-pass, no
 // cancellation); a block touching a solid fine cell is marked bad. Pass 2 (per coarse cell): the
-// filtered strain by central differences of the block means, the least-squares eddy viscosity
-// nu_t = max(0, -tau_dev : S / (2 S : S)) (coarse lattice units) and the features of the coarse
-// cell; cells on the domain faces or next to a bad block are skipped. Samples are written in cell
+// filtered strain by central differences of the block means and the least-squares numerator
+// -tau_dev : S and denominator 2 S : S. Pass 3: the target nu_t = max(0, sum num / sum den) over
+// the 3x3x3 neighbourhood (coarse lattice units) and the features of the coarse cell; cells on
+// the domain faces or next to a bad block are skipped. Samples are written in cell
 // order (per-block counts, host scan, ballot ranks), so a dataset is reproducible.
 // Record: 8 raw features, target, coarse cell index (as bits).
 // ---------------------------------------------------------------------------------------------
@@ -140,9 +141,10 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_fields(const typename S::T* __
   bad[c] = b ? 1 : 0;
 }
 
-// Gradient, target and features of coarse cell c; false if the cell is not usable.
-__device__ inline bool coarse_sample(const float* __restrict__ cf, const uint8_t* __restrict__ bad, const uint8_t* __restrict__ wd,
-                                     int cnx, int cny, int cnz, float nu0, int64_t c, float (&rec)[COARSE_REC]) {
+// Filtered strain of coarse cell c (central differences of the block means); false on the domain
+// faces and next to a bad block.
+__device__ inline bool coarse_strain(const float* __restrict__ cf, const uint8_t* __restrict__ bad, int cnx, int cny, int cnz, int64_t c,
+                                     float (&S)[6]) {
   int x, y, z;
   cell_coords(c, cnx, cny, x, y, z);
   if (x < 1 || y < 1 || z < 1 || x >= cnx - 1 || y >= cny - 1 || z >= cnz - 1) return false;
@@ -154,17 +156,50 @@ __device__ inline bool coarse_sample(const float* __restrict__ cf, const uint8_t
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int j = 0; j < 3; j++) g[i][j] = 0.5f * (cf[(c + st[i]) * 9 + j] - cf[(c - st[i]) * 9 + j]);
-  const float sxx = g[0][0], syy = g[1][1], szz = g[2][2];
-  const float sxy = 0.5f * (g[0][1] + g[1][0]), sxz = 0.5f * (g[0][2] + g[2][0]), syz = 0.5f * (g[1][2] + g[2][1]);
-  const float* t = cf + c * 9 + 3;
-  const float tr3 = (t[0] + t[1] + t[2]) * (1.0f / 3.0f);
-  const float ss = sxx * sxx + syy * syy + szz * szz + 2.0f * (sxy * sxy + sxz * sxz + syz * syz);
-  const float num = -((t[0] - tr3) * sxx + (t[1] - tr3) * syy + (t[2] - tr3) * szz + 2.0f * (t[3] * sxy + t[4] * sxz + t[5] * syz));
-  const float nut = fmaxf(0.0f, num / (2.0f * ss + 1e-30f));
+  S[0] = g[0][0]; S[1] = g[1][1]; S[2] = g[2][2];
+  S[3] = 0.5f * (g[0][1] + g[1][0]); S[4] = 0.5f * (g[0][2] + g[2][0]); S[5] = 0.5f * (g[1][2] + g[2][1]);
+  return true;
+}
+
+// Per coarse cell: the least-squares numerator -tau_dev : S and denominator 2 S : S of the eddy
+// viscosity (both 0 where the strain is not defined), for the neighbourhood average of the target.
+__global__ __launch_bounds__(BLOCK) void k_coarse_nd(const float* __restrict__ cf, const uint8_t* __restrict__ bad, int cnx, int cny,
+                                                     int cnz, float2* __restrict__ nd) {
+  const int64_t nc = (int64_t)cnx * cny * cnz;
+  const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (c >= nc) return;
+  float S[6];
+  float2 r = make_float2(0.0f, 0.0f);
+  if (coarse_strain(cf, bad, cnx, cny, cnz, c, S)) {
+    const float* t = cf + c * 9 + 3;
+    const float tr3 = (t[0] + t[1] + t[2]) * (1.0f / 3.0f);
+    r.x = -((t[0] - tr3) * S[0] + (t[1] - tr3) * S[1] + (t[2] - tr3) * S[2] + 2.0f * (t[3] * S[3] + t[4] * S[4] + t[5] * S[5]));
+    r.y = 2.0f * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + 2.0f * (S[3] * S[3] + S[4] * S[4] + S[5] * S[5]));
+  }
+  nd[c] = r;
+}
+
+// Record of coarse cell c: features from its filtered strain, wall distance and speed; target
+// nu_t = max(0, sum num / sum den) over the 3x3x3 neighbourhood (the least-squares eddy viscosity
+// of the neighbourhood, Germano-style averaging); false if the cell is not usable.
+__device__ inline bool coarse_sample(const float* __restrict__ cf, const uint8_t* __restrict__ bad, const float2* __restrict__ nd,
+                                     const uint8_t* __restrict__ wd, int cnx, int cny, int cnz, float nu0, int64_t c,
+                                     float (&rec)[COARSE_REC]) {
+  float S[6];
+  if (!coarse_strain(cf, bad, cnx, cny, cnz, c, S)) return false;
+  float num = 0.0f, den = 0.0f;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const float2 v = nd[c + dx + (int64_t)cnx * (dy + (int64_t)cny * dz)];
+        num += v.x;
+        den += v.y;
+      }
+  const float nut = fmaxf(0.0f, num / (den + 1e-30f));
   const float* ub = cf + c * 9;
   const float um = sqrtf(ub[0] * ub[0] + ub[1] * ub[1] + ub[2] * ub[2]);
   float xf[MLP_IN];
-  mlp_raw_features(sxx, syy, szz, sxy, sxz, syz, (float)wd[c] * (1.0f / 32.0f), um, nu0, xf);
+  mlp_raw_features(S[0], S[1], S[2], S[3], S[4], S[5], (float)wd[c] * (1.0f / 32.0f), um, nu0, xf);
 #pragma unroll
   for (int k = 0; k < MLP_IN; k++) rec[k] = xf[k];
   rec[8] = nut;
@@ -173,28 +208,29 @@ __device__ inline bool coarse_sample(const float* __restrict__ cf, const uint8_t
 }
 
 __global__ __launch_bounds__(BLOCK) void k_coarse_count(const float* __restrict__ cf, const uint8_t* __restrict__ bad,
-                                                        const uint8_t* __restrict__ wd, int cnx, int cny, int cnz, float nu0,
-                                                        uint32_t* __restrict__ counts) {
+                                                        const float2* __restrict__ nd, const uint8_t* __restrict__ wd, int cnx, int cny,
+                                                        int cnz, float nu0, uint32_t* __restrict__ counts) {
   __shared__ uint32_t s_n;
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
   const int64_t nc = (int64_t)cnx * cny * cnz;
   const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   float rec[COARSE_REC];
-  const bool ok = c < nc && coarse_sample(cf, bad, wd, cnx, cny, cnz, nu0, c, rec);
+  const bool ok = c < nc && coarse_sample(cf, bad, nd, wd, cnx, cny, cnz, nu0, c, rec);
   if (ok) atomicAdd(&s_n, 1u);
   __syncthreads();
   if (threadIdx.x == 0) counts[blockIdx.x] = s_n;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_coarse_samples(const float* __restrict__ cf, const uint8_t* __restrict__ bad,
-                                                          const uint8_t* __restrict__ wd, int cnx, int cny, int cnz, float nu0,
-                                                          const uint64_t* __restrict__ offsets, float* __restrict__ out) {
+                                                          const float2* __restrict__ nd, const uint8_t* __restrict__ wd, int cnx,
+                                                          int cny, int cnz, float nu0, const uint64_t* __restrict__ offsets,
+                                                          float* __restrict__ out) {
   __shared__ uint32_t s_wave[BLOCK / 64];
   const int64_t nc = (int64_t)cnx * cny * cnz;
   const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   float rec[COARSE_REC];
-  const bool ok = c < nc && coarse_sample(cf, bad, wd, cnx, cny, cnz, nu0, c, rec);
+  const bool ok = c < nc && coarse_sample(cf, bad, nd, wd, cnx, cny, cnz, nu0, c, rec);
   const uint64_t mask = __ballot(ok);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) s_wave[w] = (uint32_t)__popcll(mask);
@@ -220,16 +256,17 @@ void launch_coarse_fields(hipStream_t st, int prec, const void* f, const uint32_
   else hipLaunchKernelGGL((k_coarse_fields<SBF16>), g, dim3(BLOCK), 0, st, (const uint16_t*)f, flags, fnx, fny, fnz, cf, bad);
 }
 
-void launch_coarse_count(hipStream_t st, const float* cf, const uint8_t* bad, const uint8_t* wd, int cnx, int cny, int cnz, float nu0,
-                         uint32_t* counts) {
+void launch_coarse_count(hipStream_t st, const float* cf, const uint8_t* bad, float2* nd, const uint8_t* wd, int cnx, int cny, int cnz,
+                         float nu0, uint32_t* counts) {
   const int64_t nc = (int64_t)cnx * cny * cnz;
-  hipLaunchKernelGGL(k_coarse_count, dim3(blocks_for(nc)), dim3(BLOCK), 0, st, cf, bad, wd, cnx, cny, cnz, nu0, counts);
+  hipLaunchKernelGGL(k_coarse_nd, dim3(blocks_for(nc)), dim3(BLOCK), 0, st, cf, bad, cnx, cny, cnz, nd);
+  hipLaunchKernelGGL(k_coarse_count, dim3(blocks_for(nc)), dim3(BLOCK), 0, st, cf, bad, nd, wd, cnx, cny, cnz, nu0, counts);
 }
 
-void launch_coarse_samples(hipStream_t st, const float* cf, const uint8_t* bad, const uint8_t* wd, int cnx, int cny, int cnz, float nu0,
-                           const uint64_t* offsets, float* out) {
+void launch_coarse_samples(hipStream_t st, const float* cf, const uint8_t* bad, const float2* nd, const uint8_t* wd, int cnx, int cny,
+                           int cnz, float nu0, const uint64_t* offsets, float* out) {
   const int64_t nc = (int64_t)cnx * cny * cnz;
-  hipLaunchKernelGGL(k_coarse_samples, dim3(blocks_for(nc)), dim3(BLOCK), 0, st, cf, bad, wd, cnx, cny, cnz, nu0, offsets, out);
+  hipLaunchKernelGGL(k_coarse_samples, dim3(blocks_for(nc)), dim3(BLOCK), 0, st, cf, bad, nd, wd, cnx, cny, cnz, nu0, offsets, out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -269,7 +306,7 @@ __global__ __launch_bounds__(TB) void k_mlp_grad(const float* __restrict__ data,
       x[i] = (rec[i] - norm.mu[i]) * norm.isig[i];
       X[t][i] = x[i];
     }
-    const float target = fminf(rec[8] / norm.nu_scale, 100.0f);
+    const float target = fminf(rec[8] / norm.nu_scale, MLP_TARGET_CLIP);
     float h1[MLP_H];
 #pragma unroll
     for (int o = 0; o < MLP_H; o++) {

@@ -42,10 +42,10 @@ void launch_wall_distance(hipStream_t st, const uint32_t* flags, uint32_t* tmp, 
 int64_t coarse_blocks(int64_t nc);
 void launch_coarse_fields(hipStream_t st, int prec, const void* f, const uint32_t* flags, int fnx, int fny, int fnz, float* cf,
                           uint8_t* bad);
-void launch_coarse_count(hipStream_t st, const float* cf, const uint8_t* bad, const uint8_t* wd, int cnx, int cny, int cnz, float nu0,
-                         uint32_t* counts);
-void launch_coarse_samples(hipStream_t st, const float* cf, const uint8_t* bad, const uint8_t* wd, int cnx, int cny, int cnz, float nu0,
-                           const uint64_t* offsets, float* out);
+void launch_coarse_count(hipStream_t st, const float* cf, const uint8_t* bad, float2* nd, const uint8_t* wd, int cnx, int cny, int cnz,
+                         float nu0, uint32_t* counts);
+void launch_coarse_samples(hipStream_t st, const float* cf, const uint8_t* bad, const float2* nd, const uint8_t* wd, int cnx, int cny,
+                           int cnz, float nu0, const uint64_t* offsets, float* out);
 int64_t grad_partial_floats(int64_t samples);
 void launch_mlp_grad(hipStream_t st, const float* data, const uint32_t* perm, int64_t first, int64_t count, float inv_batch,
                      const float* params, const MlpNorm& norm, float* partials, float* grad, double* lossacc);

@@ -21,13 +21,17 @@ constexpr int MLP_NPARAM = 1377;
 
 // Feature scaling: invariants of degree k are compressed as sign(x) log(1 + |x| / S_REF^k).
 constexpr float MLP_S_REF = 1e-3f;
+// Targets are clipped at MLP_TARGET_CLIP x nu_scale in training; predictions are capped there too.
+constexpr float MLP_TARGET_CLIP = 10.0f;
+
 
 // Input normalisation and output scale that travel with the weights.
 struct MlpNorm {
   float mu[MLP_IN];
   float isig[MLP_IN];  // 1 / sigma
-  float nu_scale;      // nu_t = nu_scale * softplus(z)
-  float pad[3];
+  float nu_scale;      // nu_t = nu_scale * min(softplus(z), MLP_TARGET_CLIP)
+  float smag_floor;    // nu_t >= smag_floor * Smagorinsky's nu_t (the guard)
+  float pad[2];
 };
 
 __host__ __device__ inline float slog(float x, float ref) {
@@ -64,7 +68,8 @@ __host__ __device__ inline float softplus(float z) { return z > 20.0f ? z : log1
 // groups of 32, in transposed form so that no activation is moved between lanes:
 //   H1^T (32 x 32 cells) = W1 (32 x 8) X^T (8 x 32 cells) + b1,   relu
 //   H2^T (32 x 32 cells) = W2 (32 x 32) H1^T + b2 (four K = 8 chunks),   relu
-//   z = w3 . h2 + b3 (16 hidden units per lane, the other 16 in lane l ^ 32), nu_t = s softplus(z).
+//   z = w3 . h2 + b3 (16 hidden units per lane, the other 16 in lane l ^ 32),
+//   nu_t = s min(softplus(z), MLP_TARGET_CLIP).
 // Operand layouts (checked on gfx950): A (32 x 8): lane l holds row l % 32, k = 4 (l / 32) + 0..3;
 // B (8 x 32): lane l holds column l % 32, the same k; D (32 x 32): lane l, register r holds row
 // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32. Register block c of D (r = 4c..4c+3) is therefore
@@ -122,7 +127,7 @@ __device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* 
 #pragma unroll
     for (int r = 0; r < 16; r++) z = fmaf(net->w3d[h][r], fmaxf(acc[r], 0.0f), z);
     z += __shfl_xor(z, 32);
-    if (h == 0) nu[cell] = net->norm.nu_scale * softplus(z + net->b3);
+    if (h == 0) nu[cell] = net->norm.nu_scale * fminf(softplus(z + net->b3), MLP_TARGET_CLIP);
   }
 }
 

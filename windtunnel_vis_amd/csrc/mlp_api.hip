@@ -46,6 +46,7 @@ void build_dev(const wt_mlp* net, MlpDev* d) {
     d->norm.isig[i] = net->sigma[i] > 0.0f ? 1.0f / net->sigma[i] : 1.0f;
   }
   d->norm.nu_scale = net->nu_scale;
+  d->norm.smag_floor = net->smag_floor;
 }
 
 MlpNorm host_norm(const wt_mlp* net) {
@@ -56,6 +57,7 @@ MlpNorm host_norm(const wt_mlp* net) {
     m.isig[i] = net->sigma[i] > 0.0f ? 1.0f / net->sigma[i] : 1.0f;
   }
   m.nu_scale = net->nu_scale > 0.0f ? net->nu_scale : 1.0f;
+  m.smag_floor = net->smag_floor;
   return m;
 }
 
@@ -187,18 +189,20 @@ int wt_dataset_add(wt_dataset* d, wt_solver* fine, wt_solver* coarse, int64_t* a
   const int64_t nb = coarse_blocks(nc);
   const float nu0 = (float)coarse->prm.nu;
   float* cf = nullptr;
+  float2* nd = nullptr;
   uint8_t* bad = nullptr;
   uint32_t* counts = nullptr;
   uint64_t* offs = nullptr;
   int rc = WT_OK;
   hipStream_t st = fine->stream;
   if (hipMalloc((void**)&cf, sizeof(float) * 9 * (size_t)nc) != hipSuccess || hipMalloc((void**)&bad, (size_t)nc) != hipSuccess ||
+      hipMalloc((void**)&nd, sizeof(float2) * (size_t)nc) != hipSuccess ||
       hipMalloc((void**)&counts, sizeof(uint32_t) * (size_t)nb) != hipSuccess ||
       hipMalloc((void**)&offs, sizeof(uint64_t) * (size_t)nb) != hipSuccess) {
     rc = fail(WT_E_NOMEM, "dataset temporaries");
   } else {
     launch_coarse_fields(st, fine->prec, fine->f[fine->parity], fine->flags, fine->dims.nx, fine->dims.ny, fine->dims.nz, cf, bad);
-    launch_coarse_count(st, cf, bad, coarse->wdist, cnx, cny, cnz, nu0, counts);
+    launch_coarse_count(st, cf, bad, nd, coarse->wdist, cnx, cny, cnz, nu0, counts);
     std::vector<uint32_t> hc((size_t)nb);
     hipError_t e = hipMemcpyAsync(hc.data(), counts, sizeof(uint32_t) * (size_t)nb, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -216,7 +220,7 @@ int wt_dataset_add(wt_dataset* d, wt_solver* fine, wt_solver* coarse, int64_t* a
       } else {
         e = hipMemcpyAsync(offs, ho.data(), sizeof(uint64_t) * (size_t)nb, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) {
-          launch_coarse_samples(st, cf, bad, coarse->wdist, cnx, cny, cnz, nu0, offs, d->data);
+          launch_coarse_samples(st, cf, bad, nd, coarse->wdist, cnx, cny, cnz, nu0, offs, d->data);
           e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -229,6 +233,7 @@ int wt_dataset_add(wt_dataset* d, wt_solver* fine, wt_solver* coarse, int64_t* a
     }
   }
   if (cf) (void)hipFree(cf);
+  if (nd) (void)hipFree(nd);
   if (bad) (void)hipFree(bad);
   if (counts) (void)hipFree(counts);
   if (offs) (void)hipFree(offs);
