@@ -514,13 +514,13 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
   // inputs of both cells, normalised, staged as BF16 for the wave's MFMA evaluation
   const int lane = threadIdx.x & 63;
   const bool live = x0 < NX && k != SOLID_ALL && !(k & SKIP);
-  const float nu0 = (P.tau0 - 0.5f) * (1.0f / 3.0f);
+  const float inu0 = 3.0f / (P.tau0 - 0.5f);
   auto stage = [&](const Relax& m, uint32_t wdb, int slot) {
     const float irho = 4.5f * fast_rcp(m.rho45);
     const float sc = -1.5f * m.omega * irho;  // S = -3 Pi / (2 rho tau)
     float x[MLP_IN];
     mlp_raw_features(sc * m.pxx, sc * m.pyy, sc * m.pzz, sc * m.pxy, sc * m.pxz, sc * m.pyz, (float)wdb * (1.0f / 32.0f),
-                     __builtin_sqrtf(m.ux * m.ux + m.uy * m.uy + m.uz * m.uz), nu0, x);
+                     __builtin_sqrtf(m.ux * m.ux + m.uy * m.uy + m.uz * m.uz), inu0, x);
 #pragma unroll
     for (int i = 0; i < MLP_IN; i++) {
       const float v = live ? (x[i] - net->norm.mu[i]) * net->norm.isig[i] : 0.0f;
@@ -639,18 +639,20 @@ __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename 
 
 // The step with the learned closure in the bulk (D3Q19); wall cells keep Smagorinsky.
 template <class S, int CPT, bool REG>
-__global__ __launch_bounds__(BLOCK) void k_step_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+__global__ __launch_bounds__(BLOCK, 3) void k_step_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                                                     const uint8_t* __restrict__ kind, const uint8_t* __restrict__ wdist,
                                                     const MlpDev* __restrict__ net, KParams P, StepArgs A) {
   __shared__ short feats[BLOCK / 64][128 * MLP_IN];
   __shared__ float nus[BLOCK / 64][128];
+  __shared__ MlpDev snet;
   const uint32_t b = blockIdx.x;
   if (b < A.nwb) {
     wall_cell<D3Q19, S>(fin, fout, A, P, (int64_t)b * BLOCK + threadIdx.x);
     return;
   }
+  stage_net(net, &snet);  // bulk blocks only: the whole block passes here
   const int w = threadIdx.x >> 6;
-  bulk_block<D3Q19, S, CPT, REG, true>(fin, fout, kind, P, A, b - A.nwb, wdist, net, feats[w], nus[w]);
+  bulk_block<D3Q19, S, CPT, REG, true>(fin, fout, kind, P, A, b - A.nwb, wdist, &snet, feats[w], nus[w]);
 }
 
 // One lattice step of many independent domains in one launch (sweeps): the same wall and bulk

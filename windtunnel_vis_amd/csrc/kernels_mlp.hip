@@ -199,7 +199,7 @@ __device__ inline bool coarse_sample(const float* __restrict__ cf, const uint8_t
   const float* ub = cf + c * 9;
   const float um = sqrtf(ub[0] * ub[0] + ub[1] * ub[1] + ub[2] * ub[2]);
   float xf[MLP_IN];
-  mlp_raw_features(S[0], S[1], S[2], S[3], S[4], S[5], (float)wd[c] * (1.0f / 32.0f), um, nu0, xf);
+  mlp_raw_features(S[0], S[1], S[2], S[3], S[4], S[5], (float)wd[c] * (1.0f / 32.0f), um, 1.0f / nu0, xf);
 #pragma unroll
   for (int k = 0; k < MLP_IN; k++) rec[k] = xf[k];
   rec[8] = nut;
@@ -410,10 +410,13 @@ void launch_adam(hipStream_t st, float* p, float* m, float* v, const float* g, f
 // ---------------------------------------------------------------------------------------------
 // Stand-alone inference on the step kernel's MFMA path: raw features (n x 8) -> nu_t (n).
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_mlp_infer(const MlpDev* __restrict__ net, const float* __restrict__ raw, int64_t n,
+__global__ __launch_bounds__(BLOCK) void k_mlp_infer(const MlpDev* __restrict__ gnet, const float* __restrict__ raw, int64_t n,
                                                      float* __restrict__ nu) {
   __shared__ short feats[BLOCK / 64][128 * MLP_IN];
   __shared__ float res[BLOCK / 64][128];
+  __shared__ MlpDev snet;
+  stage_net(gnet, &snet);
+  const MlpDev* net = &snet;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int64_t base = ((int64_t)blockIdx.x * (BLOCK / 64) + w) * 128;
   // every lane stages two cells (2l, 2l + 1), as the step kernel's lanes do

@@ -34,34 +34,51 @@ struct MlpNorm {
   float pad[2];
 };
 
-__host__ __device__ inline float slog(float x, float ref) {
-  const float a = log1pf(fabsf(x) / ref);
+// log(1 + y) for y >= 0: the hardware logarithm on the device (the inputs only need a few
+// significant digits: they are normalised and rounded to BF16 before the network)
+__host__ __device__ inline float log1p_fast(float y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __logf(1.0f + y);
+#else
+  return log1pf(y);
+#endif
+}
+
+// sign(x) log(1 + |x| / ref), given 1 / ref
+__host__ __device__ inline float slog(float x, float iref) {
+  const float a = log1p_fast(fabsf(x) * iref);
   return x < 0.0f ? -a : a;
 }
 
 // The eight raw (unnormalised) inputs of a cell: six rotation invariants of the strain rate S
 // (I1 = tr S, I2 = (I1^2 - tr S^2) / 2, I3 = det S, tr S^2, tr S^3, |S| = sqrt(2 tr S^2)), the wall
-// distance d (cells, capped at 4) and the local Reynolds number |u| max(d, 1/2) / nu0.
+// distance d (cells, capped at 4) and the local Reynolds number |u| max(d, 1/2) / nu0 (inu0 = 1 / nu0).
 __host__ __device__ inline void mlp_raw_features(float sxx, float syy, float szz, float sxy, float sxz, float syz, float wd,
-                                                 float umag, float nu0, float (&x)[MLP_IN]) {
+                                                 float umag, float inu0, float (&x)[MLP_IN]) {
   const float i1 = sxx + syy + szz;
   const float tr2 = sxx * sxx + syy * syy + szz * szz + 2.0f * (sxy * sxy + sxz * sxz + syz * syz);
   const float i2 = 0.5f * (i1 * i1 - tr2);
   const float i3 = sxx * (syy * szz - syz * syz) - sxy * (sxy * szz - syz * sxz) + sxz * (sxy * syz - syy * sxz);
   const float tr3 = i1 * i1 * i1 - 3.0f * i1 * i2 + 3.0f * i3;  // Cayley-Hamilton
   const float m = sqrtf(2.0f * tr2);
-  const float r = MLP_S_REF, r2 = r * r, r3 = r2 * r;
-  x[0] = slog(i1, r);
-  x[1] = slog(i2, r2);
-  x[2] = slog(i3, r3);
-  x[3] = slog(tr2, r2);
-  x[4] = slog(tr3, r3);
-  x[5] = slog(m, r);
+  constexpr float ir = 1.0f / MLP_S_REF, ir2 = ir * ir, ir3 = ir2 * ir;
+  x[0] = slog(i1, ir);
+  x[1] = slog(i2, ir2);
+  x[2] = slog(i3, ir3);
+  x[3] = slog(tr2, ir2);
+  x[4] = slog(tr3, ir3);
+  x[5] = slog(m, ir);
   x[6] = wd;
-  x[7] = log1pf(umag * fmaxf(wd, 0.5f) / nu0);
+  x[7] = log1p_fast(umag * fmaxf(wd, 0.5f) * inu0);
 }
 
-__host__ __device__ inline float softplus(float z) { return z > 20.0f ? z : log1pf(expf(z)); }
+__host__ __device__ inline float softplus(float z) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return z > 20.0f ? z : __logf(1.0f + __expf(z));
+#else
+  return z > 20.0f ? z : log1pf(expf(z));
+#endif
+}
 
 // ---------------------------------------------------------------------------------------------
 // Wave-wide BF16 MFMA inference (v_mfma_f32_32x32x8bf16_1k). A wave evaluates 128 cells in four
@@ -96,9 +113,10 @@ struct MlpDev {
   MlpNorm norm;
 };
 
-// Evaluates nu_t for the 128 cells of a wave. feats: this wave's LDS staging area of 128 x 8 BF16
-// (cell-major; every lane wrote its own cells before the call), nu: 128 floats of LDS for the
-// results. All 64 lanes must call it (MFMA uses the whole wave); cells are indexed 0..127.
+// Evaluates nu_t for the 128 cells of a wave. net: the network (best an LDS copy, see
+// stage_net); feats: this wave's LDS staging area of 128 x 8 BF16 (cell-major; every lane wrote
+// its own cells before the call); nu: 128 floats of LDS for the results. All 64 lanes must call
+// it (MFMA uses the whole wave); cells are indexed 0..127.
 __device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* feats, float* nu) {
   const int l = threadIdx.x & 63;
   const int h = l >> 5;
@@ -129,6 +147,15 @@ __device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* 
     z += __shfl_xor(z, 32);
     if (h == 0) nu[cell] = net->norm.nu_scale * fminf(softplus(z + net->b3), MLP_TARGET_CLIP);
   }
+}
+
+// Block-wide copy of the network into LDS (every thread of the block must call it).
+__device__ inline void stage_net(const MlpDev* __restrict__ src, MlpDev* dst) {
+  constexpr int words = (int)(sizeof(MlpDev) / 4);
+  const uint32_t* a = (const uint32_t*)src;
+  uint32_t* b = (uint32_t*)dst;
+  for (int i = threadIdx.x; i < words; i += blockDim.x) b[i] = a[i];
+  __syncthreads();
 }
 
 }  // namespace wt
