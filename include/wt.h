@@ -206,6 +206,25 @@ int wt_upload_populations(wt_solver* s, const float* f);
  * EMA means meanA (4 per cell) and meanB (1 per cell). Any pointer may be NULL. */
 int wt_read_macro(wt_solver* s, float* vel4, float* mean_a4, float* mean_b);
 
+/* ---- derived fields and export ----------------------------------------------------------------
+ * Derived pass of encodePost(derived = true) (SolverGPU.ts:242-246, derivedWGSL wgslLbm.ts:527-592)
+ * over the fields of the last sample (wt_step_and_sample / wt_sample_raw): vorticity and the
+ * Q-criterion (faded out within two cells of a solid) from central differences of the velocity
+ * record, turbulence intensity sqrt(max(meanB - |meanA.xyz|^2, 0) / 3) / max(uRef, 1e-6) and a copy
+ * of the EMA mean, each stored as four IEEE halves per cell (the reference's rgba16float textures).
+ * The buffers (24 bytes per cell) are allocated, zeroed, on first use. */
+int wt_derive(wt_solver* s);
+/* Full-resolution readback of the derived fields, 4 floats per cell each: (wx, wy, wz, Q),
+ * (TI, solid, 0, 0), mean (ux, uy, uz, rho). Any pointer may be NULL. */
+int wt_read_derived(wt_solver* s, float* vort4, float* stat4, float* mean4);
+/* SolverGPU.exportFields (SolverGPU.ts:374-425): output dims ceil(n / stride) per axis, point g
+ * samples cell min(g * stride, n - 1); vel4 (ux, uy, uz, rho), mean4 (EMA mean), vort4 (wx, wy, wz, Q),
+ * 4 floats per point, x fastest (FieldExport, solver/types.ts:45-55). Reads the fields as the last
+ * sample and derived pass left them. With all three pointers NULL only out_dims is written.
+ * wt_export_dims: the output dims alone. stride < 1 is WT_E_ARG. */
+int wt_export_dims(const wt_solver* s, int32_t stride, int32_t* out_dims);
+int wt_export_fields(wt_solver* s, int32_t stride, float* vel4, float* mean4, float* vort4, int32_t* out_dims);
+
 /* ---- profiling -------------------------------------------------------------------------- */
 
 /* When enabled, every step records HIP events around its step kernel (one launch per step: wall

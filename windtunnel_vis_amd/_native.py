@@ -119,6 +119,10 @@ SIGNATURES = {
     "wt_read_populations": (C.c_int, [_P, _P]),
     "wt_upload_populations": (C.c_int, [_P, _P]),
     "wt_read_macro": (C.c_int, [_P, _P, _P, _P]),
+    "wt_derive": (C.c_int, [_P]),
+    "wt_read_derived": (C.c_int, [_P, _P, _P, _P]),
+    "wt_export_dims": (C.c_int, [_P, C.c_int32, _P]),
+    "wt_export_fields": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     "wt_profile_enable": (C.c_int, [_P, C.c_int32]),
     "wt_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "wt_work_split": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -16,7 +16,7 @@ LIB = os.path.join(LIB_DIR, "libwt.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-SOURCES = ["kernels_lbm.hip", "kernels_vox.hip", "kernels_mlp.hip", "wt_api.hip", "mlp_api.hip"]
+SOURCES = ["kernels_lbm.hip", "kernels_vox.hip", "kernels_mlp.hip", "kernels_fields.hip", "wt_api.hip", "mlp_api.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are disabled for all device code

@@ -37,6 +37,12 @@ void launch_pack(const LaunchCtx& c, const float* in, void* f);
 void launch_copy(hipStream_t st, const void* a, void* b, int64_t bytes);
 void launch_clear_stats(const LaunchCtx& c, void* meanA, float* meanB);
 
+// ---- derived fields and export (kernels_fields.hip); fields are 4 halves per cell ----
+void launch_derived(const LaunchCtx& c, const void* vel, const void* meanA, const float* meanB, void* vort, void* stat, void* mean);
+void launch_export(hipStream_t st, const void* vel, const void* mean, const void* vort, int nx, int ny, int nz, int stride,
+                   const int (&o)[3], float* ov, float* om, float* ow);
+void launch_unpack_h4(hipStream_t st, const void* in, int64_t n, float* out);
+
 // ---- learned closure (kernels_mlp.hip) ----
 void launch_wall_distance(hipStream_t st, const uint32_t* flags, uint32_t* tmp, uint8_t* out, int nx, int ny, int nz);
 int64_t coarse_blocks(int64_t nc);

@@ -44,6 +44,10 @@ struct wt_solver {
   float4* meanA = nullptr;
   float* meanB = nullptr;
   double* partials = nullptr;
+  // derived fields (allocated on first use): (wx, wy, wz, Q), (TI, solid, 0, 0), mean (ux, uy, uz, rho)
+  ushort4* vort = nullptr;
+  ushort4* stat = nullptr;
+  ushort4* meanh = nullptr;
   double* result = nullptr;
   double* h_result = nullptr;  // pinned
   int64_t nwall = 0, nfast = 0, nsolid = 0;  // cells: wall kernel, bulk fluid, bulk solid

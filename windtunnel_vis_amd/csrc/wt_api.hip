@@ -332,7 +332,7 @@ int wt_destroy(wt_solver* s) {
   if (!s) return WT_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto e : s->ev) (void)hipEventDestroy(e);
-  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->wdist, s->mlp, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result};
+  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->wdist, s->mlp, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result, s->vort, s->stat, s->meanh};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->h_result) (void)hipHostFree(s->h_result);
@@ -540,6 +540,83 @@ int wt_read_macro(wt_solver* s, float* vel4, float* mean_a4, float* mean_b) {
   if (mean_a4) HIP_TRY(hipMemcpyAsync(mean_a4, s->meanA, 16 * s->n, hipMemcpyDeviceToHost, s->stream));
   if (mean_b) HIP_TRY(hipMemcpyAsync(mean_b, s->meanB, 4 * s->n, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+// ---- derived fields and export -------------------------------------------------------------
+
+namespace wt {
+// The three derived-field buffers start zeroed (the reference's textures before the first pass).
+static int ensure_derived(wt_solver* s) {
+  if (s->vort) return WT_OK;
+  const size_t b = 8 * (size_t)s->n;
+  HIP_TRY(hipMalloc(&s->vort, b));
+  HIP_TRY(hipMalloc(&s->stat, b));
+  HIP_TRY(hipMalloc(&s->meanh, b));
+  HIP_TRY(hipMemsetAsync(s->vort, 0, b, s->stream));
+  HIP_TRY(hipMemsetAsync(s->stat, 0, b, s->stream));
+  HIP_TRY(hipMemsetAsync(s->meanh, 0, b, s->stream));
+  return WT_OK;
+}
+}  // namespace wt
+
+int wt_derive(wt_solver* s) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (int r = wt::ensure_derived(s)) return r;
+  wt::launch_derived(s->ctx(), s->vel, s->meanA, s->meanB, s->vort, s->stat, s->meanh);
+  HIP_TRY(hipGetLastError());
+  return WT_OK;
+}
+
+int wt_read_derived(wt_solver* s, float* vort4, float* stat4, float* mean4) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (int r = wt::ensure_derived(s)) return r;
+  float* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, 16 * (size_t)s->n));
+  const ushort4* src[3] = {s->vort, s->stat, s->meanh};
+  float* dst[3] = {vort4, stat4, mean4};
+  int rc = WT_OK;
+  for (int k = 0; k < 3 && rc == WT_OK; k++) {
+    if (!dst[k]) continue;
+    wt::launch_unpack_h4(s->stream, src[k], s->n, tmp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(dst[k], tmp, 16 * (size_t)s->n, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = fail(WT_E_HIP, std::string("wt_read_derived: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int wt_export_dims(const wt_solver* s, int32_t stride, int32_t* out_dims) {
+  if (!s || !out_dims) return fail(WT_E_ARG, "null argument");
+  if (stride < 1) return fail(WT_E_ARG, "stride must be >= 1");
+  out_dims[0] = (int32_t)((s->dims.nx + stride - 1) / stride);
+  out_dims[1] = (int32_t)((s->dims.ny + stride - 1) / stride);
+  out_dims[2] = (int32_t)((s->dims.nz + stride - 1) / stride);
+  return WT_OK;
+}
+
+int wt_export_fields(wt_solver* s, int32_t stride, float* vel4, float* mean4, float* vort4, int32_t* out_dims) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  int32_t o[3];
+  if (int r = wt_export_dims(s, stride, o)) return r;
+  if (out_dims) for (int k = 0; k < 3; k++) out_dims[k] = o[k];
+  if (!vel4 && !mean4 && !vort4) return WT_OK;
+  if (int r = wt::ensure_derived(s)) return r;
+  const int64_t nout = (int64_t)o[0] * o[1] * o[2];
+  float* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, 48 * (size_t)nout));
+  const int oi[3] = {o[0], o[1], o[2]};
+  wt::launch_export(s->stream, s->vel, s->meanh, s->vort, s->dims.nx, s->dims.ny, s->dims.nz, stride, oi, tmp, tmp + 4 * nout,
+                    tmp + 8 * nout);
+  hipError_t e = hipGetLastError();
+  float* dst[3] = {vel4, mean4, vort4};
+  for (int k = 0; k < 3 && e == hipSuccess; k++)
+    if (dst[k]) e = hipMemcpyAsync(dst[k], tmp + 4 * k * nout, 16 * (size_t)nout, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(WT_E_HIP, std::string("wt_export_fields: ") + hipGetErrorString(e));
   return WT_OK;
 }
 

@@ -46,6 +46,16 @@ class SolverStats:
     step: int
 
 
+@dataclass
+class FieldExport:
+    """FieldExport (src/solver/types.ts:45-55): arrays of shape (points, 4), x fastest."""
+    dims: tuple
+    stride: int
+    vel: np.ndarray   # (ux, uy, uz, rho)
+    mean: np.ndarray  # time-averaged (ux, uy, uz, rho)
+    vort: np.ndarray  # (wx, wy, wz, Q)
+
+
 DEFAULTS = dict(u=0.05, nu=0.01, cs=0.1, ground="freeslip", sides="freeslip", spongeNu=0.0, spongeStart=0.9,
                 spongeIn=0.0, tauWall=0.5, precision="f32", interp=True, collision="regularized",
                 closure="smagorinsky", emaAlpha=0.0)
@@ -225,6 +235,28 @@ class SolverGPU:
         mb = np.empty(self.n, dtype=np.float32)
         N.check(self._lib.wt_read_macro(self._h, N.ptr(vel), N.ptr(ma), N.ptr(mb)))
         return vel, ma, mb
+
+    # -- derived fields and export ------------------------------------------------------------
+    def derive(self):
+        """Derived pass of encodePost(derived=True) (SolverGPU.ts:242-246, wgslLbm.ts:527-592) over
+        the fields of the last sample: vorticity + Q-criterion, turbulence intensity, mean copy."""
+        N.check(self._lib.wt_derive(self._h))
+
+    def readDerived(self):
+        """(vort (n, 4) = wx, wy, wz, Q; stat (n, 4) = TI, solid, 0, 0; mean (n, 4)) as float32."""
+        out = [np.empty((self.n, 4), dtype=np.float32) for _ in range(3)]
+        N.check(self._lib.wt_read_derived(self._h, *(N.ptr(a) for a in out)))
+        return tuple(out)
+
+    def exportFields(self, stride):
+        """SolverGPU.exportFields (SolverGPU.ts:374-425): the velocity, mean and vorticity fields
+        sub-sampled by `stride` in every direction."""
+        dims = (N.C.c_int32 * 3)()
+        N.check(self._lib.wt_export_dims(self._h, int(stride), dims))
+        m = int(dims[0]) * int(dims[1]) * int(dims[2])
+        vel, mean, vort = (np.empty((m, 4), dtype=np.float32) for _ in range(3))
+        N.check(self._lib.wt_export_fields(self._h, int(stride), N.ptr(vel), N.ptr(mean), N.ptr(vort), dims))
+        return FieldExport((int(dims[0]), int(dims[1]), int(dims[2])), int(stride), vel, mean, vort)
 
     # -- profiling --------------------------------------------------------------------------
     def profileEnable(self, on=True):
