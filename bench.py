@@ -90,6 +90,39 @@ def run_gpu(dims, steps, warmup, precision="f16", chunk=None):
                 unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes, tune=tune)
 
 
+def precision_stress(dims=(1024, 512, 512), steps=20, warmup=5):
+    """The sports-wedge setup (300x90x45 cells, corner radius 10, u = 0.1) at 1024x512x512 in each
+    storage format: MLUPS and the step kernel's HBM roofline fraction (autotuned launch)."""
+    from windtunnel_vis_amd import SolverGPU, tuning
+    from windtunnel_vis_amd import geometry as G
+    out = {"grid": list(dims), "body": "sports wedge 300x90x45, corner 10", "u": 0.1}
+    for prec in ("f32", "f16", "bf16"):
+        s = SolverGPU(dims, workload_params(u=0.1, precision=prec))
+        body = G.sports_wedge(dims)
+        info = s.voxelizeSdf(body.prims())
+        tune = tuning.autotune(s, steps=10)
+        s.step(warmup)
+        s.profileEnable(True)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        wall = time.perf_counter() - t0
+        k_ms, launches, _ = s.profileRead()
+        st = s.stepAndSample(0)
+        sb = 4 if prec == "f32" else 2
+        fluid = s.n - info["solidCells"]
+        k_avg = k_ms / 1e3 / max(launches, 1)
+        mlups = s.n * steps / wall / 1e6
+        out[prec] = {"mlups": round(mlups, 1), "ms_per_step": round(wall * 1e3 / steps, 4),
+                     "avg_launch_ms": round(k_avg * 1e3, 4),
+                     "kernel_frac": round(fluid * 2 * 19 * sb / k_avg / 1e9 / HBM_PEAK_GBS, 4),
+                     "step_frac": round(mlups * 1e6 * 2 * 19 * sb / (HBM_PEAK_GBS * 1e9), 4),
+                     "chunk": tune["chunk"], "cells_per_thread": tune["cellsPerThread"], "unstable": bool(st.unstable)}
+        s.destroy()
+    return out
+
+
 def copy_probe(nbytes=2 << 30, iters=5):
     from windtunnel_vis_amd import _native as N
     lib = N.load()
@@ -239,6 +272,7 @@ def main():
     ap.add_argument("--north-star", action="store_true", help="also time 512x256x256")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched 24-domain sweep")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure study")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 1024x512x512 precision stress")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: copy probe + the main workload only")
     ap.add_argument("--chunk", type=int, default=None,
@@ -249,7 +283,7 @@ def main():
         sys.exit(2)
     dims = tuple(int(v) for v in args.dims.split(","))
     if args.profile_run:
-        args.no_physics = args.no_cpu = args.no_sweep = args.no_closure = True
+        args.no_physics = args.no_cpu = args.no_sweep = args.no_closure = args.no_stress = True
         args.north_star = False
     copy_gbs = copy_probe(256 << 20, 3) if args.profile_run else copy_probe()
     r = run_gpu(dims, args.steps, args.warmup, args.precision, chunk=args.chunk)
@@ -301,6 +335,8 @@ def main():
         out["batched_sweep"] = sweep_bench()
     if not args.no_closure:
         out["learned_closure"] = closure_bench()
+    if not args.no_stress:
+        out["precision_stress_1024x512x512"] = precision_stress()
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         steps = args.cpu_steps or 100

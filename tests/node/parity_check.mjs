@@ -13,7 +13,11 @@ async function main() {
   const f = await s.readPopulations();
   let sum = 0;
   for (const v of f) sum += v;
-  console.log(JSON.stringify({ stats: st, sum, first: Array.from(f.slice(0, 8)), stepCount: s.stepCount, precision: s.precision }));
+  s.derive();
+  const e = await s.exportFields(2);
+  const esum = (a) => { let t = 0; for (const v of a) t += v; return t; };
+  const ex = { dims: e.dims, vel: esum(e.vel), mean: esum(e.mean), vort: esum(e.vort), vort8: Array.from(e.vort.slice(0, 8)) };
+  console.log(JSON.stringify({ stats: st, sum, first: Array.from(f.slice(0, 8)), stepCount: s.stepCount, precision: s.precision, ex }));
   s.destroy();
 }
 main().catch((e) => { console.error(e); process.exit(2); });

@@ -31,6 +31,12 @@ def test_node_matches_python_host():
     assert o["stats"]["fx"] == st.fx and o["stats"]["fy"] == st.fy
     assert np.array_equal(np.array(o["first"], dtype=np.float32), f.reshape(-1)[:8])
     assert abs(o["sum"] - float(f.astype(np.float64).sum())) < 1e-3
+    s.derive()
+    e = s.exportFields(2)
+    assert tuple(o["ex"]["dims"]) == e.dims
+    np.testing.assert_array_equal(np.array(o["ex"]["vort8"], dtype=np.float32), e.vort.reshape(-1)[:8])
+    for k, a in (("vel", e.vel), ("mean", e.mean), ("vort", e.vort)):
+        assert abs(o["ex"][k] - float(a.astype(np.float64).sum())) <= 1e-9 * max(1.0, float(np.abs(a).sum()))
 
 
 def test_node_harness_freestream_and_cylinder():

@@ -23,7 +23,7 @@ def test_addon_exports():
     keys = set(json.loads(r.stdout))
     for k in ["create", "destroy", "setParams", "reset", "clearStats", "uploadFlags", "readFlags", "uploadSdf",
               "setWheels", "step", "stepAndSample", "readPopulations", "uploadPopulations", "voxelizeSdf",
-              "voxelizeMesh", "bytesFor"]:
+              "voxelizeMesh", "bytesFor", "derive", "exportFields"]:
         assert k in keys
 
 
@@ -38,14 +38,43 @@ def test_host_module_surface_and_errors():
     o = json.loads(r.stdout)
     assert {"SolverGPU", "solverFactory", "D2Q9", "D3Q19"} <= set(o["keys"])
     for k in ["stepAndSample", "encodeSteps", "uploadFlags", "uploadSdf", "setWheels", "readPopulations", "readFlags",
-              "setParams", "reset", "clearStats", "destroy"]:
+              "setParams", "reset", "clearStats", "destroy", "derive", "exportFields"]:
         assert k in o["proto"]
     assert o["bytes"] == 256 * 128 * 128 * (19 * 8 + 56)
     assert "dims" in o["err"]
 
 
 def test_js_files_parse_under_installed_node():
-    for f in ["solver.mjs", "analysis.mjs", "validation.mjs", "harness.mjs"]:
+    for f in ["solver.mjs", "analysis.mjs", "validation.mjs", "harness.mjs", "export.mjs"]:
         r = subprocess.run([NODE, "--check", os.path.join(ROOT, "windtunnel_vis_amd", "node", f)], stdout=subprocess.PIPE,
                            stderr=subprocess.PIPE, text=True)
         assert r.returncode == 0, (f, r.stderr)
+
+
+def test_node_writers_match_python_writers(tmp_path):
+    """The Node host's csv() / vtk() and the Python host's produce the same bytes."""
+    import numpy as np
+    from windtunnel_vis_amd import export as E
+    from windtunnel_vis_amd.solver import FieldExport
+    rng = np.random.default_rng(11)
+    dims, stride = (5, 4, 3), 2
+    m = 60
+    arrs = [rng.normal(0.05, 0.02, (m, 4)).astype(np.float32) for _ in range(3)]
+    for a in arrs[:2]:
+        a[:, 3] = 1 + rng.normal(0, 1e-3, m)
+    f = FieldExport(dims, stride, *arrs)
+    np.concatenate([a.reshape(-1) for a in arrs]).astype("<f4").tofile(str(tmp_path / "f.bin"))
+    rows = [[1, 0.123456789, "a"], [2.5e-7, float("nan"), "b"], [1234567.5, 6.02214076e23, "c"]]
+    js_rows = "[[1,0.123456789,'a'],[2.5e-7,NaN,'b'],[1234567.5,6.02214076e23,'c']]"
+    code = ("const fs=require('fs');import('./windtunnel_vis_amd/node/export.mjs').then(m=>{"
+            "const b=fs.readFileSync(%r);const a=new Float32Array(b.buffer,b.byteOffset,b.length/4);"
+            "const f={dims:[5,4,3],stride:2,vel:a.subarray(0,240),mean:a.subarray(240,480),vort:a.subarray(480,720)};"
+            "fs.writeFileSync(%r,m.vtk(f,{dx:0.0125,U:0.06,L:48,rhoRef:1.0003,title:'truck'}));"
+            "fs.writeFileSync(%r,m.csv(['t','cd','label'],%s,['run 1']));"
+            "console.log(m.strideFor(512*256*256,1e6));})"
+            % (str(tmp_path / "f.bin"), str(tmp_path / "f.vtk"), str(tmp_path / "f.csv"), js_rows))
+    r = _node(code)
+    assert r.returncode == 0, r.stderr
+    assert int(r.stdout) == E.stride_for(512 * 256 * 256, 1e6)
+    assert (tmp_path / "f.vtk").read_bytes() == E.vtk(f, dx=0.0125, U=0.06, L=48.0, rhoRef=1.0003, title="truck")
+    assert (tmp_path / "f.csv").read_text() == E.csv(["t", "cd", "label"], rows, ["run 1"])

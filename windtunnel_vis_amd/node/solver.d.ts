@@ -33,6 +33,15 @@ export interface WheelDef { c: number[]; axis: number[]; r: number; hw: number; 
 
 export interface VoxelInfo { frontal: number; solidCells: number; min: [number, number, number]; max: [number, number, number]; }
 
+/** Down-sampled field snapshot (src/solver/types.ts:45-55): 4 floats per point, x fastest. */
+export interface FieldExport {
+  dims: [number, number, number];
+  stride: number;
+  vel: Float32Array;   // (ux, uy, uz, rho)
+  mean: Float32Array;  // time-averaged (ux, uy, uz, rho)
+  vort: Float32Array;  // (wx, wy, wz, Q)
+}
+
 export interface LbmSolver {
   readonly dims: GridDims;
   readonly n: number;
@@ -68,6 +77,8 @@ export declare class SolverGPU implements LbmSolver {
   uploadPopulations(f: Float32Array): void;
   voxelizeSdf(prims: Float32Array): VoxelInfo;
   voxelizeMesh(mesh: { positions: ArrayLike<number>; indices: ArrayLike<number> }, matrix: ArrayLike<number>, wheels?: WheelDef[]): VoxelInfo;
+  derive(): void;
+  exportFields(stride: number): Promise<FieldExport>;
   destroy(): void;
   static bytesFor(dims: GridDims): number;
 }

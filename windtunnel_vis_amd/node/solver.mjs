@@ -119,6 +119,14 @@ export class SolverGPU {
       Float32Array.from(matrix), wheelArray(wheels));
   }
 
+  /** Derived pass of encodePost(derived = true) (SolverGPU.ts:242-246) over the last sample's fields. */
+  derive() { native.derive(this.h); }
+  /** SolverGPU.exportFields (SolverGPU.ts:374-425): fields sub-sampled by `stride`, FieldExport. */
+  async exportFields(stride) {
+    const r = native.exportFields(this.h, stride);
+    return { dims: r.dims, stride, vel: r.vel, mean: r.mean, vort: r.vort };
+  }
+
   destroy() {
     if (this.h) native.destroy(this.h);
     this.h = null;

@@ -478,6 +478,54 @@ napi_value VoxelizeMesh(napi_env env, napi_callback_info info) {
   return voxel_object(env, vi);
 }
 
+// derive(h): derived pass over the last sample's fields (vorticity, Q, TI, mean copy)
+napi_value Derive(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  int rc = wt_derive(h->s);
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+
+// exportFields(h, stride) -> {dims: [ox, oy, oz], vel, mean, vort} (Float32Array, 4 per point)
+napi_value ExportFields(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  int32_t stride = 0;
+  napi_get_value_int32(env, argv[1], &stride);
+  int32_t o[3];
+  int rc = wt_export_dims(h->s, stride, o);
+  if (rc) return throw_wt(env, rc);
+  const size_t m = (size_t)o[0] * o[1] * o[2];
+  void *dv, *dm, *dw;
+  napi_value vel = make_typed(env, napi_float32_array, 4 * m, 4, &dv);
+  napi_value mean = make_typed(env, napi_float32_array, 4 * m, 4, &dm);
+  napi_value vort = make_typed(env, napi_float32_array, 4 * m, 4, &dw);
+  if (!vel || !mean || !vort) {
+    napi_throw_error(env, nullptr, "exportFields: out of memory");
+    return nullptr;
+  }
+  rc = wt_export_fields(h->s, stride, (float*)dv, (float*)dm, (float*)dw, o);
+  if (rc) return throw_wt(env, rc);
+  napi_value out, dims;
+  napi_create_object(env, &out);
+  napi_create_array_with_length(env, 3, &dims);
+  for (uint32_t k = 0; k < 3; k++) {
+    napi_value x;
+    napi_create_int32(env, o[k], &x);
+    napi_set_element(env, dims, k, x);
+  }
+  napi_set_named_property(env, out, "dims", dims);
+  napi_set_named_property(env, out, "vel", vel);
+  napi_set_named_property(env, out, "mean", mean);
+  napi_set_named_property(env, out, "vort", vort);
+  return out;
+}
+
 napi_value BytesFor(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!args(env, info, 1, argv)) return nullptr;
@@ -499,6 +547,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"stepAndSample", StepAndSample}, {"stepAndSampleSync", StepAndSampleSync},
       {"readPopulations", ReadPopulations}, {"uploadPopulations", UploadPopulations},
       {"voxelizeSdf", VoxelizeSdf}, {"voxelizeMesh", VoxelizeMesh}, {"bytesFor", BytesFor},
+      {"derive", Derive}, {"exportFields", ExportFields},
   };
   for (auto& f : fns) {
     napi_value fn;
