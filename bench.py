@@ -229,6 +229,8 @@ def closure_bench():
         return {"cd": round(x["cd_mean"], 4) if x["cd_mean"] == x["cd_mean"] else None, "converged": x["converged"],
                 "converged_s": None if x["converged_wall"] is None else round(x["converged_wall"], 3),
                 "within_2pct_of_fine_s": None if x["within2_wall"] is None else round(x["within2_wall"], 3),
+                "within_band_of_fine_s": {"%d%%" % round(100 * b): None if v is None else round(v, 3)
+                                          for b, v in x["band_wall"].items()},
                 "mlups": round(x["mlups"], 1), "unstable": x["unstable"]}
 
     return {"body": "truck + trailer", "fine": run(r["fine"]), "coarse_plain": run(r["coarse_plain"]),

@@ -42,12 +42,17 @@ def make(dims, scale, nu, precision="f16"):
     return s, body, info
 
 
+BANDS = (0.02, 0.05, 0.10, 0.15)
+
+
 def run_case(s, info, L, U, max_steps, chunk=250, target=None, on_chunk=None):
-    """Steps `s` in chunks, recording C_D; returns the series and the convergence times."""
+    """Steps `s` in chunks, recording C_D; returns the series and the convergence times (for the
+    2 % band and, as a coarser picture, for every band of BANDS)."""
     t, cd, cl = [], [], []
     wall = 0.0
     conv_step = conv_wall = None
     hit_step = hit_wall = None
+    band_wall = {b: None for b in BANDS}
     step = 0
     unstable = False
     while step < max_steps:
@@ -72,11 +77,17 @@ def run_case(s, info, L, U, max_steps, chunk=250, target=None, on_chunk=None):
             hit_step, hit_wall = step, wall
         elif not inside:
             hit_step = hit_wall = None
+        for b in BANDS:
+            if ok and target is not None and abs(w["mean"] - target) <= b * abs(target):
+                if band_wall[b] is None:
+                    band_wall[b] = wall
+            else:
+                band_wall[b] = None
         if on_chunk:
             on_chunk(step)
     ok, w = converged(t, cd, L, U)
     return dict(steps=step, wall=wall, cd_mean=w["mean"], cd_se=w["se"], converged=ok, converged_step=conv_step,
-                converged_wall=conv_wall, within2_step=hit_step, within2_wall=hit_wall, unstable=unstable,
+                converged_wall=conv_wall, within2_step=hit_step, within2_wall=hit_wall, band_wall=band_wall, unstable=unstable,
                 mlups=s.n * step / max(wall, 1e-9) / 1e6)
 
 
