@@ -282,10 +282,12 @@ struct BulkIn {
 struct RowPos {
   uint32_t row;
   int iy, iz, x0;
+  const int64_t* pull;  // the pull table (source offsets per row class, then i * n)
 };
 
 __device__ __forceinline__ RowPos item_pos(uint32_t item, const StepArgs& A, const KParams& P, int cpt) {
   RowPos r;
+  r.pull = A.pull;
   r.row = A.dwpr.div(item);
   const uint32_t chunk = item - r.row * A.wpr;
   const uint32_t z = P.divy.div(r.row);
@@ -318,25 +320,20 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
   }
   const bool yLo = iy == 0, yHi = iy == NY - 1;
   const bool zLo = D3 && iz == 0, zHi = D3 && iz == NZ - 1;
-  // per direction: source direction, row and x shift (wave-uniform, branch-free)
+  // per direction the source is the row's element plus the class's offset (wave-uniform scalars)
+  const int cls = (int)yLo | ((int)yHi << 1) | ((int)zLo << 2) | ((int)zHi << 3);
+  const int64_t* __restrict__ off = r.pull + cls * Q;
+  const T* rowp = fin + (int64_t)r.row * NX;
   const T* src[Q];
   int xlim[Q];  // largest x a lane may read (outlet clamp for CPT = 1)
 #pragma unroll
   for (int i = 0; i < Q; i++) {
-    const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
-    const bool yOut = (cy > 0 && yLo) || (cy < 0 && yHi);
-    const bool zOut = (cz > 0 && zLo) || (cz < 0 && zHi);
-    const bool grnd = cy > 0 && yOut && P.ground != 0;
-    const bool yz = D3 && yOut && zOut;
-    const bool zper = P.sideMode == 1;
-    // source direction: opp (ground), mirrorYZ, mirrorY, i / mirrorZ (z faces), i
-    const int d = grnd ? opp<L>(i) : yz ? mirYZ<L>(i) : yOut ? mirY<L>(i) : (zOut && !zper) ? mirZ<L>(i) : i;
-    const int sy = (grnd || yOut) ? iy : iy - cy;
-    const int szp = D3 ? ((iz - cz + NZ) % NZ) : 0;
-    const int sz = (grnd || yz) ? iz : yOut ? (D3 ? min(max(iz - cz, 0), NZ - 1) : 0) : zOut ? (zper ? szp : iz) : iz - cz;
-    const int xs = grnd ? 0 : -cx;
-    src[i] = fin + (int64_t)d * n + ((int64_t)sy + (int64_t)NY * sz) * NX + xs;
-    xlim[i] = (grnd || cx >= 0) ? NX - 1 : NX - 2;
+    src[i] = rowp + off[i];
+    if constexpr (CPT == 1) {
+      const int cx = L::c(i, 0), cy = L::c(i, 1);
+      const bool grnd = cy > 0 && yLo && P.ground != 0;  // bounce-back pulls from the cell itself
+      xlim[i] = (grnd || cx >= 0) ? NX - 1 : NX - 2;
+    }
   }
   // all loads in flight together (scalar base + 32-bit lane byte offset)
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
@@ -364,7 +361,7 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
   auto store = [&](int i, float a, float b) {
-    T* o = (T*)((char*)(fout + uniform_i64((int64_t)i * n + rowoff)) + lane_off);
+    T* o = (T*)((char*)(fout + rowoff + r.pull[PULL_CLASSES * Q + i]) + lane_off);
     if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
     else S::st_dev(o, 0, L::w(i), a);
   };
@@ -468,7 +465,7 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
   auto store = [&](int i, float a, float b) {
-    T* o = (T*)((char*)(fout + uniform_i64((int64_t)i * n + rowoff)) + lane_off);
+    T* o = (T*)((char*)(fout + rowoff + r.pull[PULL_CLASSES * Q + i]) + lane_off);
     if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
     else S::st_dev(o, 0, L::w(i), a);
   };
@@ -937,10 +934,29 @@ void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* m
                        meta, count, cap, c.P, *c.W);
 }
 
+int pull_table_len(int q) { return (PULL_CLASSES + 1) * q; }
+
+void pull_table(const KParams& P, int64_t* out) {
+  auto go = [&](auto l) {
+    using L = decltype(l);
+    const int64_t n = P.n, nx = P.nx, nxy = (int64_t)P.nx * P.ny;
+    for (int cls = 0; cls < PULL_CLASSES; cls++)
+      for (int i = 0; i < L::Q; i++) {
+        int d, dy, dz, dx;
+        pull_source<L>(i, (cls & 1) != 0, (cls & 2) != 0, (cls & 4) != 0, (cls & 8) != 0, P.ground, P.sideMode, P.nz, d, dy, dz, dx);
+        out[cls * L::Q + i] = (int64_t)d * n + (int64_t)dy * nx + (int64_t)dz * nxy + dx;
+      }
+    for (int i = 0; i < L::Q; i++) out[PULL_CLASSES * L::Q + i] = (int64_t)i * n;
+  };
+  if (P.q == 9) go(D2Q9{});
+  else go(D3Q19{});
+}
+
 StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
                    int64_t chunk) {
   StepArgs A{};
   const int cpt = cells_per_thread(c.P);
+  A.pull = c.pull;
   A.wlist = list;
   A.wmeta = meta;
   A.wlink = links;

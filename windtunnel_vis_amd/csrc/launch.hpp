@@ -15,7 +15,12 @@ struct LaunchCtx {
   const KWheels* W;
   const uint8_t* wdist;     // wall distance bytes (learned closure), may be null
   const MlpDev* mlp;        // learned closure network (device), null: Smagorinsky
+  const int64_t* pull;      // pull table (device), see pull_source
 };
+
+// The bulk pull table of a parameter set: pull_table_len(q) offsets (host memory).
+int pull_table_len(int q);
+void pull_table(const KParams& P, int64_t* out);
 
 void launch_init(const LaunchCtx& c, void* f);
 void launch_classify(const LaunchCtx& c, uint32_t* list, uint32_t* masks, unsigned long long* counts);

@@ -215,8 +215,36 @@ struct alignas(16) WallLink {
 // per wall cell: meta = WM_* bits | x << 8
 enum : uint32_t { WM_SOLID = 1u, WM_WALL = 2u, WM_GWALL = 4u };
 
+// Bulk pull sources. A row of cells (fixed y, z) belongs to one of PULL_CLASSES classes,
+// cls = yLo | yHi << 1 | zLo << 2 | zHi << 3 (the faces it touches); in a class every direction
+// pulls from a fixed source direction and a fixed (dy, dz, dx) shift, which the face rules of the
+// pull step decide (wgslLbm.ts:139-170: ground bounce-back, y-z edge mirror, y mirror, z mirror or
+// periodic wrap, plain pull). The pull table holds, per class and direction, the element offset of
+// the pulled value relative to the pulling cell (d * n + dy * nx + dz * nx * ny + dx), then one
+// more row with i * n (the stores). Built on the host (pull_table) for each parameter set.
+constexpr int PULL_CLASSES = 16;
+
+template <class L>
+__host__ __device__ inline void pull_source(int i, bool yLo, bool yHi, bool zLo, bool zHi, int ground, int sideMode, int nz, int& d,
+                                            int& dy, int& dz, int& dx) {
+  constexpr bool D3 = L::DIM == 3;
+  const int cx = L::c(i, 0), cy = L::c(i, 1), cz = L::c(i, 2);
+  const bool yOut = (cy > 0 && yLo) || (cy < 0 && yHi);
+  const bool zOut = D3 && ((cz > 0 && zLo) || (cz < 0 && zHi));
+  const bool grnd = cy > 0 && yOut && ground != 0;
+  const bool yz = yOut && zOut;
+  const bool zper = sideMode == 1;
+  d = grnd ? opp<L>(i) : yz ? mirYZ<L>(i) : yOut ? mirY<L>(i) : (zOut && !zper) ? mirZ<L>(i) : i;
+  dy = (grnd || yOut) ? 0 : -cy;
+  // z: unchanged for ground / edge pulls; a y-mirrored pull keeps the plain z shift (it stays in
+  // the domain when the z face is not crossed); a crossed z face mirrors (0) or wraps
+  dz = (grnd || yz) ? 0 : yOut ? (D3 ? -cz : 0) : zOut ? (zper ? (cz > 0 ? nz - 1 : -(nz - 1)) : 0) : -cz;
+  dx = grnd ? 0 : -cx;
+}
+
 // Launch layout of one step (see k_step): wall blocks first, then bulk blocks of 4 waves.
 struct StepArgs {
+  const int64_t* pull;     // pull table: (PULL_CLASSES + 1) x Q element offsets (see pull_source)
   const uint32_t* wlist;   // wall cells
   const uint32_t* wmeta;   // their meta words
   const WallLink* wlink;   // their links, direction-major: wlink[(i - 1) * wcap + t]

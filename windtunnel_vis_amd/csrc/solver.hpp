@@ -67,6 +67,8 @@ struct wt_solver {
   uint8_t* wdist = nullptr;     // wall distance per cell (cells x 32, capped at 4 cells); built on demand
   bool wdist_valid = false;
   wt::MlpDev* mlp = nullptr;        // network in the MFMA layout (device)
+  int64_t* pull = nullptr;          // bulk pull table (device), rebuilt with the parameters
+  std::vector<int64_t> h_pull;      // its host copy
 
   wt::LaunchCtx ctx() const {
     wt::LaunchCtx c;
@@ -79,6 +81,7 @@ struct wt_solver {
     c.W = &W;
     c.wdist = wdist;
     c.mlp = mlp;
+    c.pull = pull;
     return c;
   }
 };

@@ -78,8 +78,15 @@ int run_init(wt_solver* s) {
   return WT_OK;
 }
 
-// Links of the wall cells (depend on flags, distances, wheels and parameters).
+// Links of the wall cells (depend on flags, distances, wheels and parameters) and the bulk pull
+// table (depends on the parameters).
 int prep_walls(wt_solver* s) {
+  const size_t len = (size_t)pull_table_len(s->q);
+  if (!s->pull) HIP_TRY(hipMalloc((void**)&s->pull, len * sizeof(int64_t)));
+  s->h_pull.assign(len, 0);
+  pull_table(s->P, s->h_pull.data());
+  HIP_TRY(hipMemcpyAsync(s->pull, s->h_pull.data(), len * sizeof(int64_t), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
   if (!s->wlink) return WT_OK;
   LaunchCtx c = s->ctx();
   launch_wallprep(c, s->wlist, s->wmask, s->wlink, s->wmeta, s->nwall, s->wcap);
@@ -332,7 +339,7 @@ int wt_destroy(wt_solver* s) {
   if (!s) return WT_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto e : s->ev) (void)hipEventDestroy(e);
-  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->wdist, s->mlp, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result, s->vort, s->stat, s->meanh};
+  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->wdist, s->mlp, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result, s->vort, s->stat, s->meanh, s->pull};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->h_result) (void)hipHostFree(s->h_result);
