@@ -18,13 +18,16 @@ __global__ __launch_bounds__(BLOCK) void k_init(typename S::T* __restrict__ fout
   const int64_t idx = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (idx >= P.n) return;
   const float U = flags[idx] != 0u ? 0.0f : P.uin;
+  const uint32_t row = P.divx.div((uint32_t)idx);
+  const int x = (int)((uint32_t)idx - row * (uint32_t)P.nx);
 #pragma unroll
   for (int i = 0; i < L::Q; i++) {
     const float cx = (float)L::c(i, 0);
+    const int64_t e = pop_at(row, i, x, L::Q, P.nx);
     if constexpr (S::BYTES == 4)
-      fout[(int64_t)i * P.n + idx] = L::w(i) * (1.0f + 3.0f * cx * U + 4.5f * cx * cx * U * U - 1.5f * U * U);
+      fout[e] = L::w(i) * (1.0f + 3.0f * cx * U + 4.5f * cx * cx * U * U - 1.5f * U * U);
     else  // half-width formats store the deviation, computed directly (wgslLbm.ts:302)
-      fout[(int64_t)i * P.n + idx] = S::enc_dev(L::w(i) * (3.0f * cx * U + 4.5f * cx * cx * U * U - 1.5f * U * U));
+      fout[e] = S::enc_dev(L::w(i) * (3.0f * cx * U + 4.5f * cx * cx * U * U - 1.5f * U * U));
   }
 }
 
@@ -98,6 +101,8 @@ __device__ void wall_links(const uint32_t* __restrict__ flags, const uint32_t* _
   const int64_t n = P.n;
   const int NX = P.nx, NY = P.ny, NZ = P.nz;
   auto cell = [&](int x, int y, int z) -> int64_t { return (int64_t)x + (int64_t)NX * ((int64_t)y + (int64_t)NY * z); };
+  // population element of direction d at cell (x, y, z)
+  auto el = [&](int d, int x, int y, int z) -> int64_t { return pop_at((int64_t)y + (int64_t)NY * z, d, x, Q, NX); };
   const float U = P.uin;
   bool wall = false, gwall = false;
   float mvSum = 0.f, mvW = 0.f;
@@ -112,7 +117,7 @@ __device__ void wall_links(const uint32_t* __restrict__ flags, const uint32_t* _
     const bool yOut = cy > 0 ? sy < 0 : (cy < 0 ? sy >= NY : false);
     const bool zOut = DIM == 3 ? (cz > 0 ? sz < 0 : (cz < 0 ? sz >= NZ : false)) : false;
     WallLink k;
-    k.e0 = (int64_t)i * n + c;
+    k.e0 = el(i, ix, iy, iz);
     k.a0 = 1.0f;
     k.a1 = 0.0f;
     k.b = 0.0f;
@@ -123,17 +128,17 @@ __device__ void wall_links(const uint32_t* __restrict__ flags, const uint32_t* _
       k.b = inlet_dev<L>(i, U);
     } else if (cy > 0 && yOut && P.ground != 0) {
       wall = true; gwall = true;
-      k.e0 = (int64_t)opp<L>(i) * n + c;
+      k.e0 = el(opp<L>(i), ix, iy, iz);
       if (cx != 0) k.b = belt * ((6.0f * w * cx) * P.uGround);
     } else {
       const int sxc = min(max(sx, 0), NX - 1);
       if (DIM == 3 && yOut && zOut) {
-        k.e0 = (int64_t)mirYZ<L>(i) * n + cell(sxc, iy, iz);
+        k.e0 = el(mirYZ<L>(i), sxc, iy, iz);
       } else if (yOut) {
-        k.e0 = (int64_t)mirY<L>(i) * n + cell(sxc, iy, DIM == 3 ? min(max(sz, 0), NZ - 1) : 0);
+        k.e0 = el(mirY<L>(i), sxc, iy, DIM == 3 ? min(max(sz, 0), NZ - 1) : 0);
       } else if (DIM == 3 && zOut) {
-        if (P.sideMode == 1) k.e0 = (int64_t)i * n + cell(sxc, sy, (sz + NZ) % NZ);
-        else k.e0 = (int64_t)mirZ<L>(i) * n + cell(sxc, sy, iz);
+        if (P.sideMode == 1) k.e0 = el(i, sxc, sy, (sz + NZ) % NZ);
+        else k.e0 = el(mirZ<L>(i), sxc, sy, iz);
       } else {
         const int64_t sc = cell(sxc, sy, sz);
         if (smask & (1u << i)) {
@@ -147,16 +152,16 @@ __device__ void wall_links(const uint32_t* __restrict__ flags, const uint32_t* _
               if (!ok) q = 0.5f;
             }
           }
-          k.e0 = (int64_t)opp<L>(i) * n + c;
+          k.e0 = el(opp<L>(i), ix, iy, iz);
           k.e1 = k.e0;
           if (q < 0.5f) {
             k.a0 = 2.0f * q;
             k.a1 = 1.0f - 2.0f * q;
-            k.e1 = (int64_t)opp<L>(i) * n + cell(ix + cx, iy + cy, DIM == 3 ? iz + cz : 0);
+            k.e1 = el(opp<L>(i), ix + cx, iy + cy, DIM == 3 ? iz + cz : 0);
           } else if (q > 0.5f) {
             k.a0 = 0.5f / q;
             k.a1 = 1.0f - 0.5f / q;
-            k.e1 = (int64_t)i * n + c;
+            k.e1 = el(i, ix, iy, iz);
           }
           const uint32_t tag = flags[sc] >> 8;
           if (tag != 0u) {
@@ -169,7 +174,7 @@ __device__ void wall_links(const uint32_t* __restrict__ flags, const uint32_t* _
             mvW += w;
           }
         } else {
-          k.e0 = (int64_t)i * n + sc;
+          k.e0 = el(i, sxc, sy, sz);
         }
       }
     }
@@ -323,7 +328,7 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
   // per direction the source is the row's element plus the class's offset (wave-uniform scalars)
   const int cls = (int)yLo | ((int)yHi << 1) | ((int)zLo << 2) | ((int)zHi << 3);
   const int64_t* __restrict__ off = r.pull + cls * Q;
-  const T* rowp = fin + (int64_t)r.row * NX;
+  const T* rowp = fin + (int64_t)r.row * Q * NX;
   const T* src[Q];
   int xlim[Q];  // largest x a lane may read (outlet clamp for CPT = 1)
 #pragma unroll
@@ -355,7 +360,7 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
   const int x0 = r.x0;
   if (x0 >= NX) return;
   const int64_t n = P.n;
-  const int64_t rowoff = (int64_t)r.row * NX;
+  const int64_t rowoff = (int64_t)r.row * Q * NX;  // the row's population block
   const uint32_t k = in.k;
   constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
@@ -459,7 +464,7 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
   const int NX = P.nx;
   const int x0 = r.x0;
   const int64_t n = P.n;
-  const int64_t rowoff = (int64_t)r.row * NX;
+  const int64_t rowoff = (int64_t)r.row * Q * NX;  // the row's population block
   const uint32_t k = in.k;
   constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
@@ -572,14 +577,16 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
   if (t >= A.nwall) return;
   const int64_t c = A.wlist[t];
   const uint32_t meta = A.wmeta[t];
-  const int64_t n = P.n;
+  const uint32_t row = P.divx.div((uint32_t)c);
+  const int64_t base = pop_at(row, 0, (int)((uint32_t)c - row * (uint32_t)P.nx), Q, P.nx);  // direction i: base + i * nx
+  const int64_t nx = P.nx;
   float f[Q];
   if (meta & WM_SOLID) {
 #pragma unroll
-    for (int i = 0; i < Q; i++) S::st_dev(fout, (int64_t)i * n + c, L::w(i), 0.0f);
+    for (int i = 0; i < Q; i++) S::st_dev(fout, base + i * nx, L::w(i), 0.0f);
     return;
   }
-  f[0] = S::ld_dev(fin, c, L::w(0));
+  f[0] = S::ld_dev(fin, base, L::w(0));
 #pragma unroll
   for (int i = 1; i < Q; i++) {
     const WallLink k = A.wlink[(int64_t)(i - 1) * A.wcap + t];
@@ -589,7 +596,7 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
   }
   collide<L>(f, (float)(meta >> 8), (meta & WM_WALL) != 0u, (meta & WM_GWALL) != 0u, P);
 #pragma unroll
-  for (int i = 0; i < Q; i++) S::st_dev(fout, (int64_t)i * n + c, L::w(i), f[i]);
+  for (int i = 0; i < Q; i++) S::st_dev(fout, base + i * nx, L::w(i), f[i]);
 }
 
 // One lattice step in one launch. The first A.nwb blocks update the wall cells (short gathers on
@@ -706,13 +713,14 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
       if (x < 0 || y < 0 || z < 0 || x >= NX || y >= NY || z >= NZ) return -1;
       return (int64_t)x + (int64_t)NX * ((int64_t)y + (int64_t)NY * z);
     };
+    auto el = [&](int d, int x, int y, int z) -> int64_t { return pop_at((int64_t)y + (int64_t)NY * z, d, x, Q, NX); };
     float4 out = make_float4(0.f, 0.f, 0.f, 1.f);
     const uint8_t kd = kind[idx];
     const float a = P.emaAlpha;
     if (!(kd & KIND_SOLID)) {
       float f[Q];
 #pragma unroll
-      for (int i = 0; i < Q; i++) f[i] = S::ld_dev(fin, (int64_t)i * n + idx, L::w(i));
+      for (int i = 0; i < Q; i++) f[i] = S::ld_dev(fin, el(i, ix, iy, iz), L::w(i));
       float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
 #pragma unroll
       for (int i = 0; i < Q; i++) {
@@ -736,7 +744,8 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
           float fr = f[i];
           if (q < 0.5f) {
             const int64_t bk = safe(ix - cx, iy - cy, iz - cz);
-            if (bk >= 0 && flags[bk] == 0u) fr = 2.0f * q * f[i] + (1.0f - 2.0f * q) * S::ld_dev(fin, (int64_t)i * n + bk, L::w(i));
+            if (bk >= 0 && flags[bk] == 0u)
+              fr = 2.0f * q * f[i] + (1.0f - 2.0f * q) * S::ld_dev(fin, el(i, ix - cx, iy - cy, iz - cz), L::w(i));
             else q = 0.5f;
           } else if (q > 0.5f) {
             fr = (0.5f / q) * f[i] + (1.0f - 0.5f / q) * f[opp<L>(i)];
@@ -776,7 +785,7 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
         if (s >= 0 && flags[s] == 0u) {
           float r = 0.f;
 #pragma unroll
-          for (int i = 0; i < Q; i++) r += S::ld_dev(fin, (int64_t)i * n + s, L::w(i));
+          for (int i = 0; i < Q; i++) r += S::ld_dev(fin, el(i, ix + nb[k][0], iy + nb[k][1], iz + nb[k][2]), L::w(i));
           rs += 1.0f + r; rn += 1.0f;
         }
       }
@@ -837,26 +846,35 @@ __global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ part
   }
 }
 
-// Convert between device storage and full-value host floats.
+// Convert between device storage and full-value host floats. The host side is direction-major,
+// out[i * n + cell]; element e of it maps to the row-blocked device element pop_at(row, i, x).
+__device__ inline int64_t host_to_dev(int64_t e, const KParams& P, int q, int& i) {
+  i = (int)(e / P.n);
+  const int64_t c = e - (int64_t)i * P.n;
+  const uint32_t row = P.divx.div((uint32_t)c);
+  return pop_at(row, i, (int)((uint32_t)c - row * (uint32_t)P.nx), q, P.nx);
+}
 template <class L, class S>
 __global__ __launch_bounds__(BLOCK) void k_unpack(const typename S::T* __restrict__ f, float* __restrict__ out, KParams P) {
   const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (e >= (int64_t)L::Q * P.n) return;
-  const int i = (int)(e / P.n);
+  int i;
+  const int64_t d = host_to_dev(e, P, L::Q, i);
   float w = 0.0f;
 #pragma unroll
   for (int k = 0; k < L::Q; k++) if (k == i) w = L::w(k);
-  out[e] = S::ld(f, e, w);
+  out[e] = S::ld(f, d, w);
 }
 template <class L, class S>
 __global__ __launch_bounds__(BLOCK) void k_pack(const float* __restrict__ in, typename S::T* __restrict__ f, KParams P) {
   const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (e >= (int64_t)L::Q * P.n) return;
-  const int i = (int)(e / P.n);
+  int i;
+  const int64_t d = host_to_dev(e, P, L::Q, i);
   float w = 0.0f;
 #pragma unroll
   for (int k = 0; k < L::Q; k++) if (k == i) w = L::w(k);
-  f[e] = S::enc(in[e], w);
+  f[d] = S::enc(in[e], w);
 }
 
 // Plain streaming copy, one 16-byte vector per lane over a grid that covers the buffer: the
@@ -939,14 +957,15 @@ int pull_table_len(int q) { return (PULL_CLASSES + 1) * q; }
 void pull_table(const KParams& P, int64_t* out) {
   auto go = [&](auto l) {
     using L = decltype(l);
-    const int64_t n = P.n, nx = P.nx, nxy = (int64_t)P.nx * P.ny;
+    const int64_t nx = P.nx;
     for (int cls = 0; cls < PULL_CLASSES; cls++)
       for (int i = 0; i < L::Q; i++) {
         int d, dy, dz, dx;
         pull_source<L>(i, (cls & 1) != 0, (cls & 2) != 0, (cls & 4) != 0, (cls & 8) != 0, P.ground, P.sideMode, P.nz, d, dy, dz, dx);
-        out[cls * L::Q + i] = (int64_t)d * n + (int64_t)dy * nx + (int64_t)dz * nxy + dx;
+        // the source cell's row is row + dy + dz * ny; its block starts (dy + dz * ny) * Q * nx later
+        out[cls * L::Q + i] = (((int64_t)dy + (int64_t)dz * P.ny) * L::Q + d) * nx + dx;
       }
-    for (int i = 0; i < L::Q; i++) out[PULL_CLASSES * L::Q + i] = (int64_t)i * n;
+    for (int i = 0; i < L::Q; i++) out[PULL_CLASSES * L::Q + i] = (int64_t)i * nx;
   };
   if (P.q == 9) go(D2Q9{});
   else go(D3Q19{});

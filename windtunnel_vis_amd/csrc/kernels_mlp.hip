@@ -104,12 +104,14 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_fields(const typename S::T* __
   bool b = false;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const int64_t e = (int64_t)(2 * x + (k & 1)) + (int64_t)fnx * ((int64_t)(2 * y + ((k >> 1) & 1)) + (int64_t)fny * (2 * z + (k >> 2)));
+    const int64_t frow = (int64_t)(2 * y + ((k >> 1) & 1)) + (int64_t)fny * (2 * z + (k >> 2));
+    const int fx = 2 * x + (k & 1);
+    const int64_t e = (int64_t)fx + (int64_t)fnx * frow;
     b |= flags[e] != 0u;
     float dr = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
 #pragma unroll
     for (int i = 0; i < L::Q; i++) {
-      const float d = S::ld_dev(f, (int64_t)i * nf + e, L::w(i));
+      const float d = S::ld_dev(f, pop_at(frow, i, fx, L::Q, fnx), L::w(i));
       dr += d;
       if (L::c(i, 0)) jx += L::c(i, 0) * d;
       if (L::c(i, 1)) jy += L::c(i, 1) * d;

