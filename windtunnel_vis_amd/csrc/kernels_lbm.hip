@@ -1,5 +1,7 @@
 // Lattice-Boltzmann kernels for gfx950: init, cell classification, the bulk fused
 // pull-stream + collide kernel, the wall kernel, and the macro / momentum-exchange reduction.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 #include "lbm.hpp"
@@ -1011,15 +1013,15 @@ void launch_step_batch(hipStream_t st, int q, int prec, int cpt, bool reg, const
   });
 }
 
-void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A) {
+void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A, hipEvent_t ev0, hipEvent_t ev1) {
   const int cpt = cells_per_thread(c.P);
   const bool reg = c.P.collision != 1;
   const dim3 grid(A.nwb + A.nbb);
   if (c.P.closure == 1 && c.mlp && c.wdist && c.P.q == 19) {
     auto go = [&](auto kern, auto s) {
       using S = decltype(s);
-      hipLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind, c.wdist, c.mlp,
-                         c.P, A);
+      hipExtLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, c.stream, ev0, ev1, 0, (const typename S::T*)fin, (typename S::T*)fout, c.kind,
+                            c.wdist, c.mlp, c.P, A);
     };
     auto pick = [&](auto s) {
       using S = decltype(s);
@@ -1038,8 +1040,10 @@ void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs
   }
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
+    // with events: hipExtLaunchKernelGGL records them at the kernel's own start and end
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, (typename S::T*)fout, c.kind, c.P, A);
+      hipExtLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, c.stream, ev0, ev1, 0, (const typename S::T*)fin, (typename S::T*)fout, c.kind,
+                            c.P, A);
     };
     if (cpt == 2) {
       if (reg) go(k_step<L, S, 2, true>);

@@ -30,7 +30,8 @@ void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* m
 // chunk: items per XCD run (<= 0: one contiguous eighth of the items per XCD)
 StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
                    int64_t chunk);
-void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A);
+// ev0 / ev1 (may be null): events recorded at the step kernel's own start and end
+void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // One step of a batch of domains sharing lattice, storage, cells per thread and collision.
 void launch_step_batch(hipStream_t st, int q, int prec, int cpt, bool reg, const DomainDesc* descs, const uint32_t* dmap,
                        uint32_t wall_total, uint32_t blocks, int parity);

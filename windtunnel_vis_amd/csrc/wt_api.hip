@@ -142,9 +142,7 @@ int do_steps(wt_solver* s, int steps) {
       }
       hipEvent_t* e = &s->ev[s->ev_used];
       s->ev_used += 2;
-      HIP_TRY(hipEventRecord(e[0], s->stream));
-      launch_step(c, fin, fout, A);
-      HIP_TRY(hipEventRecord(e[1], s->stream));
+      launch_step(c, fin, fout, A, e[0], e[1]);
     } else {
       launch_step(c, fin, fout, A);
     }
