@@ -78,3 +78,53 @@ def test_node_writers_match_python_writers(tmp_path):
     assert int(r.stdout) == E.stride_for(512 * 256 * 256, 1e6)
     assert (tmp_path / "f.vtk").read_bytes() == E.vtk(f, dx=0.0125, U=0.06, L=48.0, rhoRef=1.0003, title="truck")
     assert (tmp_path / "f.csv").read_text() == E.csv(["t", "cd", "label"], rows, ["run 1"])
+
+
+def test_node_controller_matches_python_controller():
+    """The Node host's RunController and the Python one agree frame by frame on a scripted solver
+    (forces as a function of the step count, one unstable sample)."""
+    import math
+    from windtunnel_vis_amd.controller import RunController
+    from windtunnel_vis_amd.solver import SolverStats
+
+    class Fake:
+        def __init__(self):
+            self.dims = dict(nx=64, ny=32, nz=32)
+            self.stepCount = 0
+
+        def setParams(self, p):
+            self.last = dict(p)
+
+        def reset(self):
+            self.stepCount = 0
+
+        def stepAndSample(self, n):
+            self.stepCount += n
+            s = self.stepCount
+            return SolverStats(0.02 * (1 + 0.1 * math.sin(s / 37.0)), 0.003 * math.cos(s / 53.0), 1e-4, 0.05, 1.0 + 1e-4 * math.sin(s),
+                               s == 60, s)
+
+    c = RunController(Fake(), Lcells=40, frontal=90, speed_kmh=180)
+    for _ in range(400):
+        c.frame(20)
+    py = dict(c.readout(), stab=c.stab, cur=c.cur, clock=c.clock, last=c.solver.last)
+    code = ("import('./windtunnel_vis_amd/node/controller.mjs').then(async (m) => {"
+            " const f = {dims: {nx: 64, ny: 32, nz: 32}, stepCount: 0, setParams(p) { this.last = {...p}; },"
+            " reset() { this.stepCount = 0; }, async stepAndSample(n) { this.stepCount += n; const s = this.stepCount;"
+            " return {fx: 0.02 * (1 + 0.1 * Math.sin(s / 37)), fy: 0.003 * Math.cos(s / 53), fz: 1e-4, maxU: 0.05,"
+            " rhoRef: 1 + 1e-4 * Math.sin(s), unstable: s === 60, step: s}; } };"
+            " const c = new m.RunController(f, {Lcells: 40, frontal: 90, speedKmh: 180});"
+            " for (let k = 0; k < 400; k++) await c.frame(20);"
+            " console.log(JSON.stringify({...c.readout(), stab: c.stab, cur: c.cur, clock: c.clock, last: f.last})); })")
+    r = _node(code)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    assert js["stab"] == py["stab"] and py["stab"]["events"] == 1
+    for k in ("cd", "cl", "cs", "cdInst", "cdSE", "avgSpan", "cdCorrected", "dragN", "convTime", "maxMach", "blockage"):
+        assert js[k] == pytest.approx(py[k], rel=1e-12, abs=1e-15), k
+    assert js["converged"] == py["converged"]
+    for k in ("U", "nu"):
+        assert js["cur"][k] == pytest.approx(py["cur"][k], rel=1e-12)
+    for k in ("reSim", "U", "nu", "dx", "dt"):
+        assert js["flow"][k] == pytest.approx(py["flow"][k], rel=1e-12)
+    assert js["last"]["emaAlpha"] == pytest.approx(py["last"]["emaAlpha"], rel=1e-12)

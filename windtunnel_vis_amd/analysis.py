@@ -92,3 +92,54 @@ def macroscopic(f, lattice):
     rho = f.sum(axis=0)
     u = (lattice.c.T.astype(np.float64) @ f) / rho
     return rho, u
+
+
+AIR_RHO = 1.225  # kg/m^3 (units.ts:10)
+
+
+def dragNewtons(cd, speed, areaM2):
+    """Real drag force at the chosen speed (aero.ts:10-12)."""
+    return 0.5 * AIR_RHO * speed * speed * cd * areaM2
+
+
+class Ema:
+    """Exponential moving average that starts at its first finite sample (aero.ts:15-26)."""
+
+    def __init__(self, k):
+        self.k = k
+        self.value = math.nan
+
+    def push(self, v):
+        if not math.isfinite(v):
+            return self.value
+        self.value = v if math.isnan(self.value) else self.value + self.k * (v - self.value)
+        return self.value
+
+    def reset(self):
+        self.value = math.nan
+
+
+class Series:
+    """Fixed-capacity (t, v) series that ignores non-finite values (aero.ts:55-76)."""
+
+    def __init__(self, cap=600):
+        self.cap = cap
+        self.t = []
+        self.v = []
+
+    def push(self, t, v):
+        if not math.isfinite(v):
+            return
+        self.t.append(t)
+        self.v.append(v)
+        if len(self.t) > self.cap:
+            self.t.pop(0)
+            self.v.pop(0)
+
+    def clear(self):
+        self.t = []
+        self.v = []
+
+    def meanSince(self, t0):
+        xs = [v for t, v in zip(self.t, self.v) if t >= t0]
+        return sum(xs) / len(xs) if xs else math.nan
