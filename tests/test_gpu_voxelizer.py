@@ -91,3 +91,31 @@ def test_sdf_voxelizer_matches_fp64(maker):
     if body.wheels:
         assert ((flags >> 8) > 0).sum() > 0
     s.destroy()
+
+
+@pytest.mark.parametrize("pid", ["sedan", "sports", "suv", "truck", "f1", "ahmed25", "ahmed35", "sphere", "cylinder"])
+def test_preset_meshes_bit_exact(pid):
+    """The procedural presets through the mesh voxelizer, placed as the app places them (ground
+    vehicles at a third of the tunnel length, validation bodies centred), wheels tagged."""
+    from windtunnel_vis_amd import presets as P
+    mesh = P.build_preset(pid)
+    info_p = {p["id"]: p for p in P.PRESETS}[pid]
+    dims = (128, 64, 64)
+    mode = info_p["placement"]
+    M, _ = G.placement_matrix(mesh, dims, mode=mode, ride_cells=1.0 if mode == "ground" else 0.0)
+    wheels = G.grid_wheels(mesh, M)
+    s = _solver(dims)
+    s.setWheels(wheels)
+    info = s.voxelizeMesh(mesh.positions, mesh.indices, M, wheels)
+    flags_ref, bits_ref, info_ref = voxelize_mesh(mesh.positions, mesh.indices, M, dims, wheels)
+    flags = s.readFlags()
+    print(pid, info, "wheel cells", int(((flags >> 8) != 0).sum()), flush=True)
+    assert info == info_ref
+    assert np.array_equal(flags, flags_ref), int((flags != flags_ref).sum())
+    assert np.array_equal(_sdf_bits(s), bits_ref)
+    assert info["solidCells"] > 100 and info["frontal"] > 20
+    if wheels:
+        assert ((flags >> 8) != 0).any()
+    st = s.stepAndSample(200)
+    assert not st.unstable and math.isfinite(st.fx)
+    s.destroy()
