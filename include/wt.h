@@ -225,6 +225,43 @@ int wt_read_derived(wt_solver* s, float* vort4, float* stat4, float* mean4);
 int wt_export_dims(const wt_solver* s, int32_t stride, int32_t* out_dims);
 int wt_export_fields(wt_solver* s, int32_t stride, float* vel4, float* mean4, float* vort4, int32_t* out_dims);
 
+/* ---- probes, streamlines and particle tracers (the compute passes of render/tracers.ts) --------
+ * They sample the velocity record of the last sample (and, for the mean, the derived pass's mean
+ * record) like the reference's linear, clamp-to-edge textures: texel centres at cell centres,
+ * trilinear (here in FP32), the single layer in 2D. Points are in cell units. */
+typedef struct wt_rake {  /* seeding line a -> b, parallel lines offset by d (xyz used, w ignored) */
+  float a[4], b[4], d[4];
+} wt_rake;
+/* PROBES_WGSL (tracers.ts:109-125): out8 per point = field (ux, uy, uz, rho), mean (ux, uy, uz, rho);
+ * z components 0 in 2D. pos4: 4 floats per point (xyz used). */
+int wt_probe(wt_solver* s, const float* pos4, int32_t count, float* out8);
+/* STREAMLINES_WGSL (tracers.ts:152-200): `seeds` lines seeded per_line per rake line; `points`
+ * points each, step h along the normalised velocity (midpoint rule) of the field (use_mean: the
+ * mean field); out4 = (x, y, z, speed), speed -1 once the line has left the domain, entered a
+ * solid or slowed below 1e-3 U. */
+int wt_streamlines(wt_solver* s, const wt_rake* rake, int32_t seeds, int32_t per_line, int32_t points, float h, int32_t use_mean,
+                   float* out4);
+/* PARTICLES_WGSL (tracers.ts:3-106): a particle system on the solver's velocity field, state
+ * (x, y, z, age) per particle and a ring buffer of `trail` past positions. The handle must be
+ * destroyed before its solver. */
+typedef struct wt_particles wt_particles;
+typedef struct wt_particle_opts {
+  int32_t count, trail;
+  int32_t emitter;   /* 0: inlet plane x in [1, 3), 1: rake nozzles */
+  int32_t nozzles, lines;
+  float max_age;     /* steps of life before a forced respawn: 4 x max_age frames */
+  wt_rake rake;
+} wt_particle_opts;
+int wt_particles_create(wt_solver* s, const wt_particle_opts* o, wt_particles** out);
+int wt_particles_destroy(wt_particles* p);
+/* One frame (RendererGPU.ts:505-527): fill = fresh positions (and trail); else with advance the
+ * trail head moves on and every particle advances by `steps` lattice steps. `seed` feeds the
+ * per-particle hash generator (the renderer passes a frame counter). */
+int wt_particles_update(wt_particles* p, float steps, uint32_t seed, int32_t fill, int32_t advance);
+/* state4: count x 4 floats; hist4: trail x count x 4 floats (slot-major); head: current slot.
+ * Any pointer may be NULL. */
+int wt_particles_read(wt_particles* p, float* state4, float* hist4, int32_t* head);
+
 /* ---- profiling -------------------------------------------------------------------------- */
 
 /* When enabled, every step records HIP events around its step kernel (one launch per step: wall

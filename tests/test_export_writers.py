@@ -227,3 +227,74 @@ def test_export_sampling():
                 px, py, pz = min(gx * 3, 6), min(gy * 3, 4), min(gz * 3, 3)
                 want.append(f[px + 7 * (py + 5 * pz)])
     np.testing.assert_array_equal(s, np.array(want))
+
+
+# ---- tracer oracle pins ---------------------------------------------------------------------------
+
+def _linear_field(dims, g):
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+    u = g(x.reshape(-1), y.reshape(-1), z.reshape(-1))
+    return np.concatenate([np.stack(u, axis=1), np.ones((x.size, 1))], axis=1)
+
+
+def test_sampler_reproduces_linear_fields_and_clamps():
+    dims = (8, 6, 5)
+    f = _linear_field(dims, lambda x, y, z: (0.1 + 0.01 * x, 0.02 * y - 0.003 * z, 0.004 * x * 0 + 0.005 * z))
+    rng = np.random.default_rng(4)
+    p = rng.uniform([0.5, 0.5, 0.5], [7.5, 5.5, 4.5], (50, 3))  # between the first and last texel centres
+    s = FO.sample(f, dims, p)
+    np.testing.assert_allclose(s[:, 0], 0.1 + 0.01 * p[:, 0], atol=1e-14)
+    np.testing.assert_allclose(s[:, 1], 0.02 * p[:, 1] - 0.003 * p[:, 2], atol=1e-14)
+    np.testing.assert_allclose(s[:, 2], 0.005 * p[:, 2], atol=1e-14)
+    edge = FO.sample(f, dims, [[0.1, 3.0, 2.0], [7.9, 3.0, 2.0]])  # clamp-to-edge: constant outside
+    np.testing.assert_allclose(edge[:, 0], [0.1 + 0.005, 0.1 + 0.075], atol=1e-14)
+    d2 = (6, 4, 1)
+    f2 = _linear_field(d2, lambda x, y, z: (0.01 * x, 0.02 * y, 0 * x))
+    np.testing.assert_allclose(FO.sample(f2, d2, [[2.0, 2.0, 7.0]])[0, :2], [0.02, 0.04], atol=1e-14)  # z ignored in 2D
+
+
+def test_streamlines_uniform_and_rotation():
+    dims = (32, 16, 8)
+    f = _linear_field(dims, lambda x, y, z: (0.05 + 0 * x, 0 * x, 0 * x))
+    rake = dict(a=(2.0, 4.0, 4.0), b=(2.0, 12.0, 4.0), d=(0.0, 0.0, 1.0))
+    sl = FO.streamlines(f, np.zeros(32 * 16 * 8, bool), dims, rake, 4, 2, 10, 0.5, 0.05)
+    # seeds 2 per line on 2 lines: y at 1/4, 3/4 of the rake, z offset -0.5 / +0.5
+    np.testing.assert_allclose(sl[:, 0, 1], [6.0, 10.0, 6.0, 10.0])
+    np.testing.assert_allclose(sl[:, 0, 2], [3.5, 3.5, 4.5, 4.5])
+    np.testing.assert_allclose(sl[0, :, 0], 2.0 + 0.5 * np.arange(10))  # straight lines, step h
+    np.testing.assert_allclose(sl[:, :, 3], 0.05)
+    solid = np.zeros((8, 16, 32), bool)
+    solid[:, :, 5] = True  # a wall at x = 5: the lines stop entering it
+    sl = FO.streamlines(f, solid.reshape(-1), dims, rake, 1, 1, 10, 0.5, 0.05)
+    assert (sl[0, 6:, 3] == -1).all() and (sl[0, :6, 3] > 0).all()
+    # rigid rotation: the midpoint rule on the unit direction keeps the radius to O(h^3) per step
+    dr = (40, 40, 1)
+    fr = _linear_field(dr, lambda x, y, z: (-(y - 20) * 0.002, (x - 20) * 0.002, 0 * x))
+    sl = FO.streamlines(fr, np.zeros(1600, bool), dr, dict(a=(30, 20, 0), b=(30, 20, 0), d=(0, 0, 0)), 1, 1, 60, 0.5, 0.02)
+    r = np.hypot(sl[0, :, 0] - 20, sl[0, :, 1] - 20)
+    assert np.abs(r - 10).max() < 2e-3
+
+
+def test_particles_fill_and_uniform_advection():
+    dims = (32, 16, 8)
+    f = _linear_field(dims, lambda x, y, z: (0.05 + 0 * x, 0 * x, 0 * x))
+    solid = np.zeros(32 * 16 * 8, bool)
+    o = dict(count=64, trail=4, emitter=0, nozzles=8, lines=1, max_age=400.0, rake=dict(a=(0, 0, 0), b=(0, 0, 0), d=(0, 0, 0)))
+    st = np.zeros((64, 4))
+    hist = np.zeros((4, 64, 4))
+    FO.particles_update(st, hist, 0, f, solid, dims, o, 10.0, 7, True, False)
+    assert (st[:, 0] >= 1) .all() and (st[:, 0] < 31).all() and (st[:, 3] == 4).all()
+    assert (hist == np.concatenate([st[None, :, :3], np.zeros((1, 64, 1))], axis=2)).all()
+    p0 = st.copy()
+    FO.particles_update(st, hist, 1, f, solid, dims, o, 10.0, 8, False, True)
+    moved = st[:, 0] < 31 - 0.5  # the ones that did not leave through the outlet
+    np.testing.assert_allclose(st[moved, 0], p0[moved, 0] + 0.5, atol=1e-12)  # RK2 is exact for a uniform flow
+    np.testing.assert_allclose(st[moved, 3], 5)
+    assert (hist[1, :, :3] == st[:, :3]).all()
+    # inlet spawn: x in [1, 3)
+    xs = []
+    r = FO._Rng(FO.pcg_hash(5))
+    for _ in range(200):
+        xs.append(FO._spawn(r, o, dims)[0])
+    assert min(xs) >= 1.0 and max(xs) < 3.0

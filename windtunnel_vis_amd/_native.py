@@ -3,8 +3,10 @@
 The library is loaded from the package's in-tree build (windtunnel_vis_amd/_lib/libwt.so). There
 is no fallback: if the library is missing or cannot be loaded, `load()` raises.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libwt.so")
@@ -94,6 +96,15 @@ class Prim(C.Structure):
 
 # every exported symbol of include/wt.h with its signature: (restype, argtypes)
 _P = C.c_void_p
+class Rake(C.Structure):
+    _fields_ = [("a", C.c_float * 4), ("b", C.c_float * 4), ("d", C.c_float * 4)]
+
+
+class ParticleOpts(C.Structure):
+    _fields_ = [("count", C.c_int32), ("trail", C.c_int32), ("emitter", C.c_int32), ("nozzles", C.c_int32),
+                ("lines", C.c_int32), ("max_age", C.c_float), ("rake", Rake)]
+
+
 SIGNATURES = {
     "wt_create": (C.c_int, [C.POINTER(Dims), C.POINTER(Params), C.POINTER(_P)]),
     "wt_destroy": (C.c_int, [_P]),
@@ -123,6 +134,12 @@ SIGNATURES = {
     "wt_read_derived": (C.c_int, [_P, _P, _P, _P]),
     "wt_export_dims": (C.c_int, [_P, C.c_int32, _P]),
     "wt_export_fields": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
+    "wt_probe": (C.c_int, [_P, _P, C.c_int32, _P]),
+    "wt_streamlines": (C.c_int, [_P, C.POINTER(Rake), C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int32, _P]),
+    "wt_particles_create": (C.c_int, [_P, C.POINTER(ParticleOpts), C.POINTER(_P)]),
+    "wt_particles_destroy": (C.c_int, [_P]),
+    "wt_particles_update": (C.c_int, [_P, C.c_float, C.c_uint32, C.c_int32, C.c_int32]),
+    "wt_particles_read": (C.c_int, [_P, _P, _P, C.POINTER(C.c_int32)]),
     "wt_profile_enable": (C.c_int, [_P, C.c_int32]),
     "wt_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "wt_work_split": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -185,3 +202,27 @@ def ptr(a):
     """Raw data pointer of a C-contiguous numpy array."""
     assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
     return C.c_void_p(a.ctypes.data)
+
+
+# Handles still open when the interpreter exits are destroyed by an atexit hook, newest first (a
+# tracer before its solver), while the HIP runtime is certainly alive: left to the garbage
+# collector, they could be finalized after the runtime's own teardown.
+_live = []
+
+
+def track(obj):
+    if len(_live) > 256:
+        _live[:] = [r for r in _live if r() is not None]
+    _live.append(weakref.ref(obj))
+
+
+@atexit.register
+def _close_all():
+    while _live:
+        obj = _live.pop()()
+        if obj is not None:
+            try:
+                obj.destroy()
+            except Exception:
+                pass
+

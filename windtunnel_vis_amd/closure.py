@@ -59,6 +59,7 @@ class Dataset:
         h = N.C.c_void_p()
         N.check(self._lib.wt_dataset_create(int(capacity), N.C.byref(h)))
         self._h = h
+        N.track(self)
 
     def add(self, fine, coarse):
         """Append the samples of `fine`'s current state, box-filtered onto `coarse`'s grid."""

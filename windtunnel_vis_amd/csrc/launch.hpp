@@ -1,5 +1,6 @@
 // Host-side launch interface between the C-ABI layer and the kernels.
 #pragma once
+#include "../../include/wt.h"
 #include "lbm.hpp"
 #include "mlp.hpp"
 
@@ -48,6 +49,13 @@ void launch_derived(const LaunchCtx& c, const void* vel, const void* meanA, cons
 void launch_export(hipStream_t st, const void* vel, const void* mean, const void* vort, int nx, int ny, int nz, int stride,
                    const int (&o)[3], float* ov, float* om, float* ow);
 void launch_unpack_h4(hipStream_t st, const void* in, int64_t n, float* out);
+// probes (2 float4 per point: field, mean field), streamlines (points float4 per seed), tracers
+void launch_probes(hipStream_t st, const void* vel, const void* mean, const uint8_t* kind, int nx, int ny, int nz, const float* pos,
+                   int count, float* out);
+void launch_streamlines(hipStream_t st, const void* field, const uint8_t* kind, int nx, int ny, int nz, const wt_rake& rake, int seeds,
+                        int per_line, int points, float h, float U, float* out);
+void launch_particles(hipStream_t st, const void* vel, const uint8_t* kind, int nx, int ny, int nz, const wt_particle_opts& o, int head,
+                      float steps, uint32_t seed, int fill, int advance, float* state, float* hist);
 
 // ---- learned closure (kernels_mlp.hip) ----
 void launch_wall_distance(hipStream_t st, const uint32_t* flags, uint32_t* tmp, uint8_t* out, int nx, int ny, int nz);

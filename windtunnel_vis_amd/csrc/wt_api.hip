@@ -625,6 +625,117 @@ int wt_export_fields(wt_solver* s, int32_t stride, float* vel4, float* mean4, fl
   return WT_OK;
 }
 
+// ---- probes, streamlines, particle tracers ------------------------------------------------------
+
+int wt_probe(wt_solver* s, const float* pos4, int32_t count, float* out8) {
+  if (!s || (count > 0 && (!pos4 || !out8))) return fail(WT_E_ARG, "null argument");
+  if (count < 0) return fail(WT_E_ARG, "count must be >= 0");
+  if (count == 0) return WT_OK;
+  if (int r = wt::ensure_derived(s)) return r;
+  float *dp = nullptr, *dout = nullptr;
+  if (hipMalloc((void**)&dp, 16 * (size_t)count) != hipSuccess || hipMalloc((void**)&dout, 32 * (size_t)count) != hipSuccess) {
+    if (dp) (void)hipFree(dp);
+    return fail(WT_E_NOMEM, "probe buffers");
+  }
+  hipError_t e = hipMemcpyAsync(dp, pos4, 16 * (size_t)count, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) {
+    wt::launch_probes(s->stream, s->vel, s->meanh, s->kind, s->dims.nx, s->dims.ny, s->dims.nz, dp, count, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out8, dout, 32 * (size_t)count, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(dp);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(WT_E_HIP, std::string("wt_probe: ") + hipGetErrorString(e));
+  return WT_OK;
+}
+
+int wt_streamlines(wt_solver* s, const wt_rake* rake, int32_t seeds, int32_t per_line, int32_t points, float h, int32_t use_mean,
+                   float* out4) {
+  if (!s || !rake || (seeds > 0 && points > 0 && !out4)) return fail(WT_E_ARG, "null argument");
+  if (seeds < 0 || points < 0 || per_line < 0) return fail(WT_E_ARG, "seeds, per_line and points must be >= 0");
+  if (!(h > 0.0f)) return fail(WT_E_ARG, "h must be > 0");
+  if (seeds == 0 || points == 0) return WT_OK;
+  if (use_mean) {
+    if (int r = wt::ensure_derived(s)) return r;
+  }
+  const size_t bytes = 16 * (size_t)seeds * (size_t)points;
+  float* dout = nullptr;
+  if (hipMalloc((void**)&dout, bytes) != hipSuccess) return fail(WT_E_NOMEM, "streamline buffer");
+  wt::launch_streamlines(s->stream, use_mean ? (const void*)s->meanh : (const void*)s->vel, s->kind, s->dims.nx, s->dims.ny,
+                         s->dims.nz, *rake, seeds, per_line, points, h, (float)s->prm.u, dout);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out4, dout, bytes, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(WT_E_HIP, std::string("wt_streamlines: ") + hipGetErrorString(e));
+  return WT_OK;
+}
+
+struct wt_particles {
+  wt_solver* s = nullptr;
+  wt_particle_opts o{};
+  float* state = nullptr;  // count x float4
+  float* hist = nullptr;   // trail x count x float4
+  int head = 0;
+  bool need_fill = true;
+};
+
+int wt_particles_create(wt_solver* s, const wt_particle_opts* o, wt_particles** out) {
+  if (!s || !o || !out) return fail(WT_E_ARG, "null argument");
+  if (o->count <= 0 || o->trail <= 0) return fail(WT_E_ARG, "count and trail must be > 0");
+  if (o->emitter < 0 || o->emitter > 1) return fail(WT_E_ARG, "emitter must be 0 (inlet) or 1 (rake)");
+  if ((int64_t)o->count * o->trail > ((int64_t)1 << 31)) return fail(WT_E_ARG, "count x trail too large");
+  auto* p = new wt_particles();
+  p->s = s;
+  p->o = *o;
+  if (hipMalloc((void**)&p->state, 16 * (size_t)o->count) != hipSuccess ||
+      hipMalloc((void**)&p->hist, 16 * (size_t)o->count * (size_t)o->trail) != hipSuccess) {
+    wt_particles_destroy(p);
+    return fail(WT_E_NOMEM, "particle buffers");
+  }
+  // zeroed, as freshly created buffers are in the reference
+  hipError_t e = hipMemsetAsync(p->state, 0, 16 * (size_t)o->count, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->hist, 0, 16 * (size_t)o->count * (size_t)o->trail, s->stream);
+  if (e != hipSuccess) {
+    wt_particles_destroy(p);
+    return fail(WT_E_HIP, std::string("wt_particles_create: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return WT_OK;
+}
+
+int wt_particles_destroy(wt_particles* p) {
+  if (!p) return WT_OK;
+  // hipFree waits for the device, so the solver (possibly destroyed already) is not touched here
+  if (p->state) (void)hipFree(p->state);
+  if (p->hist) (void)hipFree(p->hist);
+  delete p;
+  return WT_OK;
+}
+
+int wt_particles_update(wt_particles* p, float steps, uint32_t seed, int32_t fill, int32_t advance) {
+  if (!p) return fail(WT_E_ARG, "null argument");
+  const bool f = fill != 0 || p->need_fill;
+  if (advance && !f) p->head = (p->head + 1) % p->o.trail;
+  wt_solver* s = p->s;
+  wt::launch_particles(s->stream, s->vel, s->kind, s->dims.nx, s->dims.ny, s->dims.nz, p->o, p->head, steps, seed, f ? 1 : 0,
+                       advance ? 1 : 0, p->state, p->hist);
+  HIP_TRY(hipGetLastError());
+  p->need_fill = false;
+  return WT_OK;
+}
+
+int wt_particles_read(wt_particles* p, float* state4, float* hist4, int32_t* head) {
+  if (!p) return fail(WT_E_ARG, "null argument");
+  hipStream_t st = p->s->stream;
+  if (state4) HIP_TRY(hipMemcpyAsync(state4, p->state, 16 * (size_t)p->o.count, hipMemcpyDeviceToHost, st));
+  if (hist4) HIP_TRY(hipMemcpyAsync(hist4, p->hist, 16 * (size_t)p->o.count * (size_t)p->o.trail, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (head) *head = p->head;
+  return WT_OK;
+}
+
 int wt_profile_enable(wt_solver* s, int32_t on) {
   if (!s) return fail(WT_E_ARG, "null argument");
   HIP_TRY(hipStreamSynchronize(s->stream));

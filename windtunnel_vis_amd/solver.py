@@ -103,6 +103,7 @@ class SolverGPU:
         h = N.C.c_void_p()
         N.check(self._lib.wt_create(N.C.byref(d), N.C.byref(ps), N.C.byref(h)))
         self._h = h
+        N.track(self)
         info = self.info()
         self.precision = N.PRECISION_NAME[info.precision]
         self.packed = self.precision != "f32"
@@ -258,6 +259,24 @@ class SolverGPU:
         N.check(self._lib.wt_export_fields(self._h, int(stride), N.ptr(vel), N.ptr(mean), N.ptr(vort), dims))
         return FieldExport((int(dims[0]), int(dims[1]), int(dims[2])), int(stride), vel, mean, vort)
 
+    # -- probes, streamlines, tracers (render/tracers.ts) --------------------------------------
+    def probe(self, points):
+        """Field and mean field at points (k, 3) in cell units: returns (vel (k, 4), mean (k, 4))."""
+        p = np.zeros((len(points), 4), np.float32)
+        p[:, :3] = np.asarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.empty((len(p), 8), np.float32)
+        N.check(self._lib.wt_probe(self._h, N.ptr(p), len(p), N.ptr(out)))
+        return out[:, :4], out[:, 4:]
+
+    def streamlines(self, rake, seeds, points, h=0.5, perLine=None, useMean=False):
+        """(seeds, points, 4) array of (x, y, z, speed or -1) along streamlines seeded on `rake`
+        (dict a, b, d of 3 floats; perLine seeds per rake line, default all on one line)."""
+        r = rake_struct(rake)
+        out = np.empty((seeds, points, 4), np.float32)
+        N.check(self._lib.wt_streamlines(self._h, N.C.byref(r), int(seeds), int(perLine or seeds), int(points), float(h),
+                                         1 if useMean else 0, N.ptr(out)))
+        return out
+
     # -- profiling --------------------------------------------------------------------------
     def profileEnable(self, on=True):
         N.check(self._lib.wt_profile_enable(self._h, 1 if on else 0))
@@ -293,6 +312,59 @@ class SolverGPU:
         return a.value, b.value
 
 
+def rake_struct(rake):
+    r = N.Rake()
+    for k in ("a", "b", "d"):
+        v = list(rake.get(k, (0.0, 0.0, 0.0)))[:3] + [0.0]
+        getattr(r, k)[:] = [float(x) for x in v]
+    return r
+
+
+class Tracers:
+    """Particle tracers on a solver's velocity field (PARTICLES_WGSL, render/tracers.ts:3-106)."""
+
+    def __init__(self, solver, count, trail=16, emitter="inlet", nozzles=16, lines=1, maxAge=400.0, rake=None):
+        self.solver = solver
+        self.count, self.trail = int(count), int(trail)
+        o = N.ParticleOpts()
+        o.count, o.trail = self.count, self.trail
+        o.emitter = 0 if emitter == "inlet" else 1
+        o.nozzles, o.lines, o.max_age = int(nozzles), int(lines), float(maxAge)
+        o.rake = rake_struct(rake or {})
+        self._lib = solver._lib
+        h = N.C.c_void_p()
+        N.check(self._lib.wt_particles_create(solver._h, N.C.byref(o), N.C.byref(h)))
+        self._h = h
+        N.track(self)
+        self.frameSeed = 0
+
+    def update(self, steps, advance=True, fill=False, seed=None):
+        """One frame: `steps` lattice steps of advection (the renderer passes a frame counter as seed)."""
+        if seed is None:
+            seed = self.frameSeed
+            self.frameSeed += 1
+        N.check(self._lib.wt_particles_update(self._h, float(steps), int(seed) & 0xFFFFFFFF, 1 if fill else 0, 1 if advance else 0))
+
+    def read(self):
+        """(state (count, 4) = x, y, z, age; trail (trail, count, 4); head slot)."""
+        st = np.empty((self.count, 4), np.float32)
+        hist = np.empty((self.trail, self.count, 4), np.float32)
+        head = N.C.c_int32()
+        N.check(self._lib.wt_particles_read(self._h, N.ptr(st), N.ptr(hist), N.C.byref(head)))
+        return st, hist, head.value
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            N.check(self._lib.wt_particles_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 def wheel_array(wheels):
     out = []
     for w in list(wheels)[:16] if wheels is not None else []:
@@ -315,6 +387,7 @@ class BatchGPU:
         h = N.C.c_void_p()
         N.check(self._lib.wt_batch_create(arr, len(self.solvers), N.C.byref(h)))
         self._h = h
+        N.track(self)
         self.setChunk(chunk, graph)
 
     def setChunk(self, chunk, graph=True):
