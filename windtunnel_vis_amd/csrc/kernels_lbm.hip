@@ -319,11 +319,11 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
   const int64_t c0 = (int64_t)r.row * NX + x0;
   // every source address below is in bounds whatever the kind, so the loads need not wait for
   // the skip test
-  if constexpr (CPT == 2) in.k = *(const uint16_t*)(kind + c0);
-  else in.k = kind[c0];
+  if constexpr (CPT == 2) in.k = *glob((const uint16_t*)(kind + c0));
+  else in.k = glob(kind)[c0];
   if constexpr (MLP) {
-    if constexpr (CPT == 2) in.wd = *(const uint16_t*)(wdist + c0);
-    else in.wd = wdist[c0];
+    if constexpr (CPT == 2) in.wd = *glob((const uint16_t*)(wdist + c0));
+    else in.wd = glob(wdist)[c0];
   }
   const bool yLo = iy == 0, yHi = iy == NY - 1;
   const bool zLo = D3 && iz == 0, zHi = D3 && iz == NZ - 1;
@@ -335,7 +335,7 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
   int xlim[Q];  // largest x a lane may read (outlet clamp for CPT = 1)
 #pragma unroll
   for (int i = 0; i < Q; i++) {
-    src[i] = rowp + off[i];
+    src[i] = rowp + glob(off)[i];
     if constexpr (CPT == 1) {
       const int cx = L::c(i, 0), cy = L::c(i, 1);
       const bool grnd = cy > 0 && yLo && P.ground != 0;  // bounce-back pulls from the cell itself
@@ -368,7 +368,7 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
   auto store = [&](int i, float a, float b) {
-    T* o = (T*)((char*)(fout + rowoff + r.pull[PULL_CLASSES * Q + i]) + lane_off);
+    T* o = (T*)((char*)(fout + rowoff + glob(r.pull)[PULL_CLASSES * Q + i]) + lane_off);
     if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
     else S::st_dev(o, 0, L::w(i), a);
   };
@@ -472,7 +472,7 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
   auto store = [&](int i, float a, float b) {
-    T* o = (T*)((char*)(fout + rowoff + r.pull[PULL_CLASSES * Q + i]) + lane_off);
+    T* o = (T*)((char*)(fout + rowoff + glob(r.pull)[PULL_CLASSES * Q + i]) + lane_off);
     if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
     else S::st_dev(o, 0, L::w(i), a);
   };
@@ -569,6 +569,16 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
   }
 }
 
+// One 32-byte wall link, read as two 16-byte global loads.
+__device__ __forceinline__ WallLink load_link(const WallLink* p) {
+  typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+  struct alignas(16) Pair { u32x4v a, b; } v;
+  const u32x4v* q = (const u32x4v*)p;
+  v.a = *glob(q);
+  v.b = *glob(q + 1);
+  return __builtin_bit_cast(WallLink, v);
+}
+
 // Wall cells: one lane per listed cell (cells next to the body). Every pulled population is a
 // precomputed link (two independent gathers and two FMAs), so all loads of a lane are in flight
 // together; then the collision (BGK on wall cells, wgslLbm.ts:186).
@@ -577,8 +587,8 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
                                           const StepArgs& A, const KParams& P, int64_t t) {
   constexpr int Q = L::Q;
   if (t >= A.nwall) return;
-  const int64_t c = A.wlist[t];
-  const uint32_t meta = A.wmeta[t];
+  const int64_t c = glob(A.wlist)[t];
+  const uint32_t meta = glob(A.wmeta)[t];
   const uint32_t row = P.divx.div((uint32_t)c);
   const int64_t base = pop_at(row, 0, (int)((uint32_t)c - row * (uint32_t)P.nx), Q, P.nx);  // direction i: base + i * nx
   const int64_t nx = P.nx;
@@ -591,7 +601,7 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
   f[0] = S::ld_dev(fin, base, L::w(0));
 #pragma unroll
   for (int i = 1; i < Q; i++) {
-    const WallLink k = A.wlink[(int64_t)(i - 1) * A.wcap + t];
+    const WallLink k = load_link(A.wlink + (int64_t)(i - 1) * A.wcap + t);
     const float x0 = S::ld_dev(fin, k.e0, L::w(i));
     const float x1 = S::ld_dev(fin, k.e1, L::w(i));
     f[i] = fmaf(k.a0, x0, fmaf(k.a1, x1, k.b));
@@ -671,6 +681,7 @@ __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step_batch(const Dom
   const uint32_t b = blockIdx.x;
   const uint32_t d = __builtin_amdgcn_readfirstlane(dmap[b]);
   const DomainDesc& dd = D[d];
+  // (the pointers read from the table are accessed through glob(), see lbm.hpp)
   const T* fin = (const T*)dd.f[parity];
   T* fout = (T*)dd.f[parity ^ 1];
   if (b < wall_total) {

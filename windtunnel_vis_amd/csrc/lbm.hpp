@@ -19,6 +19,17 @@
 
 namespace wt {
 
+// Device pointers handed to the kernels always address global memory. Accesses go through
+// glob(), a cast to the global address space: pointers the compiler cannot trace to a kernel
+// argument (the batched step reads them from its domain table) would otherwise be accessed with
+// flat instructions, which also count against the scalar-load wait counter and serialise every
+// parameter fetch behind the population loads.
+#define WT_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const WT_GLOBAL T* glob(const T* p) { return (const WT_GLOBAL T*)p; }
+template <class T>
+__device__ __forceinline__ WT_GLOBAL T* glob(T* p) { return (WT_GLOBAL T*)p; }
+
 // Population layout, row-blocked: the Q values of one x-row of cells (row = y + ny * z) lie in Q
 // consecutive runs of nx elements, f[(row * Q + i) * nx + x]. A wave's pulls for one row then touch
 // a few neighbouring row blocks instead of Q planes n elements apart (fewer open DRAM pages; 7 %
@@ -96,10 +107,10 @@ __device__ inline float b2f(uint32_t h) { return __uint_as_float(h << 16); }
 struct SF32 {
   using T = float;
   static constexpr int BYTES = 4;
-  __device__ static float ld(const T* p, int64_t e, float) { return p[e]; }
-  __device__ static float ld_dev(const T* p, int64_t e, float w) { return p[e] - w; }
+  __device__ static float ld(const T* p, int64_t e, float) { return glob(p)[e]; }
+  __device__ static float ld_dev(const T* p, int64_t e, float w) { return glob(p)[e] - w; }
   __device__ static T enc(float f, float) { return f; }
-  __device__ static void st_dev(T* p, int64_t e, float w, float d) { p[e] = w + d; }
+  __device__ static void st_dev(T* p, int64_t e, float w, float d) { glob(p)[e] = w + d; }
   // two consecutive cells; FP32 pairs are only 4-byte aligned in general
   template <bool ALIGNED>
   __device__ static void ld2_dev(const T* p, float w, float& a, float& b) {
@@ -109,11 +120,11 @@ struct SF32 {
   }
   __device__ static void st2_dev(T* p, float w, float a, float b) {
     f32x2a4 v = {w + a, w + b};
-    __builtin_nontemporal_store(v, (f32x2a4*)p);
+    __builtin_nontemporal_store(v, glob((f32x2a4*)p));
   }
   // raw access for the bulk kernel: a word holds two consecutive cells
   using Word = uint64_t;
-  __device__ static Word ld_word(const T* p) { return *(const u64_a4*)p; }
+  __device__ static Word ld_word(const T* p) { return *glob((const u64_a4*)p); }
   __device__ static void unpack_word(Word v, float w, float& a, float& b) {
     a = __uint_as_float((uint32_t)v) - w;
     b = __uint_as_float((uint32_t)(v >> 32)) - w;
@@ -126,19 +137,19 @@ struct SF32 {
 struct SF16 {
   using T = uint16_t;
   static constexpr int BYTES = 2;
-  __device__ static float ld(const T* p, int64_t e, float w) { return h2f(p[e]) + w; }
-  __device__ static float ld_dev(const T* p, int64_t e, float) { return h2f(p[e]); }
+  __device__ static float ld(const T* p, int64_t e, float w) { return h2f(glob(p)[e]) + w; }
+  __device__ static float ld_dev(const T* p, int64_t e, float) { return h2f(glob(p)[e]); }
   __device__ static T enc(float f, float w) { return (T)(pack_h2(f - w, 0.0f) & 0xffffu); }
   __device__ static T enc_dev(float d) { return (T)(pack_h2(d, 0.0f) & 0xffffu); }
-  __device__ static void st_dev(T* p, int64_t e, float, float d) { p[e] = enc_dev(d); }
+  __device__ static void st_dev(T* p, int64_t e, float, float d) { glob(p)[e] = enc_dev(d); }
   template <bool ALIGNED>
   __device__ static void ld2_dev(const T* p, float, float& a, float& b) {
     const uint32_t v = ALIGNED ? *(const uint32_t*)p : *(const u32_a2*)p;
     unpack_h2(v, a, b);
   }
-  __device__ static void st2_dev(T* p, float, float a, float b) { __builtin_nontemporal_store(pack_h2(a, b), (uint32_t*)p); }
+  __device__ static void st2_dev(T* p, float, float a, float b) { __builtin_nontemporal_store(pack_h2(a, b), glob((uint32_t*)p)); }
   using Word = uint32_t;
-  __device__ static Word ld_word(const T* p) { return *(const u32_a2*)p; }
+  __device__ static Word ld_word(const T* p) { return *glob((const u32_a2*)p); }
   __device__ static void unpack_word(Word v, float, float& a, float& b) { unpack_h2(v, a, b); }
   __device__ static float lo_dev(Word v, float) { return (float)__builtin_bit_cast(f16x2, v).x; }
   __device__ static float hi_dev(Word v, float) { return (float)__builtin_bit_cast(f16x2, v).y; }
@@ -148,19 +159,19 @@ struct SF16 {
 struct SBF16 {
   using T = uint16_t;
   static constexpr int BYTES = 2;
-  __device__ static float ld(const T* p, int64_t e, float w) { return b2f(p[e]) + w; }
-  __device__ static float ld_dev(const T* p, int64_t e, float) { return b2f(p[e]); }
+  __device__ static float ld(const T* p, int64_t e, float w) { return b2f(glob(p)[e]) + w; }
+  __device__ static float ld_dev(const T* p, int64_t e, float) { return b2f(glob(p)[e]); }
   __device__ static T enc(float f, float w) { return (T)(pack_b2(f - w, 0.0f) & 0xffffu); }
   __device__ static T enc_dev(float d) { return (T)(pack_b2(d, 0.0f) & 0xffffu); }
-  __device__ static void st_dev(T* p, int64_t e, float, float d) { p[e] = enc_dev(d); }
+  __device__ static void st_dev(T* p, int64_t e, float, float d) { glob(p)[e] = enc_dev(d); }
   template <bool ALIGNED>
   __device__ static void ld2_dev(const T* p, float, float& a, float& b) {
     const uint32_t v = ALIGNED ? *(const uint32_t*)p : *(const u32_a2*)p;
     unpack_b2(v, a, b);
   }
-  __device__ static void st2_dev(T* p, float, float a, float b) { __builtin_nontemporal_store(pack_b2(a, b), (uint32_t*)p); }
+  __device__ static void st2_dev(T* p, float, float a, float b) { __builtin_nontemporal_store(pack_b2(a, b), glob((uint32_t*)p)); }
   using Word = uint32_t;
-  __device__ static Word ld_word(const T* p) { return *(const u32_a2*)p; }
+  __device__ static Word ld_word(const T* p) { return *glob((const u32_a2*)p); }
   __device__ static void unpack_word(Word v, float, float& a, float& b) { unpack_b2(v, a, b); }
   __device__ static float lo_dev(Word v, float) { return __uint_as_float(v << 16); }
   __device__ static float hi_dev(Word v, float) { return __uint_as_float(v & 0xffff0000u); }
