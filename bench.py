@@ -237,7 +237,10 @@ def closure_bench():
             "coarse_learned": run(r["coarse_learned"]), "cd_err_plain": round(r["cd_err_plain"], 4),
             "cd_err_learned": round(r["cd_err_learned"], 4), "training_records": r["records"],
             "train_seconds": round(r["train_seconds"], 2), "train_loss": [round(x, 5) for x in r["train_loss"]],
-            "smag_floor": r["network"].smag_floor}
+            "smag_floor": r["network"].smag_floor, "nu_scale": r["network"].nu_scale,
+            "ensemble_cd": {k: [round(x, 4) for x in v] for k, v in r["ensemble"].items()},
+            "ensemble_note": "coarse runs from seeded perturbed starts (1 % of U): their spread is the scatter of one "
+                             "converged-window C_D"}
 
 
 def cpu_baseline(dims, sample_steps, threads):

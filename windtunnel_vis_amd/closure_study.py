@@ -34,12 +34,35 @@ def study_params(nu, precision="f16"):
                 tauWall=0.53, interp=True, collision="regularized", emaAlpha=0.02, precision=precision)
 
 
-def make(dims, scale, nu, precision="f16"):
+def make(dims, scale, nu, precision="f16", seed=None):
+    """The truck run; with `seed`, the start state carries a seeded velocity perturbation."""
     s = SolverGPU(dims, study_params(nu, precision))
     body = truck(dims, scale)
     s.setWheels(body.wheels)
     info = s.voxelizeSdf(body.prims())
+    if seed is not None:
+        perturb(s, 0.06, seed)
     return s, body, info
+
+
+def perturb(s, U, seed, amp=1e-2):
+    """Equilibrium start at rho = 1, u = (U, 0, 0) + amp U (uniform noise per component and fluid
+    cell, seeded); solid cells at rest. Members of an ensemble differ only in `seed`."""
+    rng = np.random.default_rng(seed)
+    flags = s.readFlags()
+    lat = s.lattice
+    n = s.n
+    u = np.zeros((3, n), np.float32)
+    u[0] = U
+    u += (amp * U) * rng.uniform(-1.0, 1.0, (3, n)).astype(np.float32)
+    u[:, flags != 0] = 0.0
+    c = lat.c.T.astype(np.float32)  # (3, q)
+    usq = (u * u).sum(axis=0)
+    f = np.empty((lat.q, n), np.float32)
+    for i in range(lat.q):
+        cu = c[0, i] * u[0] + c[1, i] * u[1] + c[2, i] * u[2]
+        f[i] = lat.w[i] * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usq)
+    s.uploadPopulations(f)
 
 
 BANDS = (0.02, 0.05, 0.10, 0.15)
@@ -93,7 +116,7 @@ def run_case(s, info, L, U, max_steps, chunk=250, target=None, on_chunk=None):
 
 def study(fine_dims=(512, 256, 256), coarse_dims=(256, 128, 128), nu_coarse=6e-4, fine_steps=30000, coarse_steps=16000,
           snapshot_from=12000, snapshot_every=3000, train_samples=10_000_000, epochs=6, batch=8192, lr=2e-3, seed=20261203,
-          precision="f16", smag_floor=1.0, log=None):
+          precision="f16", smag_floor=1.0, ensemble=3, log=None):
     say = log or (lambda *a: None)
     U = 0.06
     scale_f = fine_dims[0] / 512.0
@@ -130,13 +153,24 @@ def study(fine_dims=(512, 256, 256), coarse_dims=(256, 128, 128), nu_coarse=6e-4
     rl = run_case(learned, linfo, L_c, U, coarse_steps, target=ref)
     say("coarse learned C_D %.4f" % rl["cd_mean"])
     learned.destroy()
+    # the same two coarse runs from seeded perturbed starts: the spread of the converged C_D over
+    # the members is the run-to-run scatter of the measurement itself (a chaotic wake)
+    ens = dict(plain=[], learned=[])
+    for k in range(ensemble):
+        for kind in ("plain", "learned"):
+            s, _, info = make(coarse_dims, scale_f / 2, nu_coarse, precision, seed=seed + 1 + k)
+            if kind == "learned":
+                closure.attach(s, net)
+            ens[kind].append(run_case(s, info, L_c, U, coarse_steps, target=ref)["cd_mean"])
+            s.destroy()
+    say("ensemble C_D plain %s learned %s" % (np.round(ens["plain"], 4).tolist(), np.round(ens["learned"], 4).tolist()))
 
     def err(r):
         return abs(r["cd_mean"] - ref) / abs(ref) if math.isfinite(r["cd_mean"]) else math.inf
 
     out = dict(fine_dims=list(fine_dims), coarse_dims=list(coarse_dims), cd_fine=ref, fine=rf, coarse_plain=rp, coarse_learned=rl,
                cd_err_plain=err(rp), cd_err_learned=err(rl), records=n_records, snapshots=snaps, train_seconds=t_train,
-               train_loss=losses.tolist(), network=net)
+               train_loss=losses.tolist(), network=net, ensemble=ens)
     # time to the fine run's C_D (2 %) against the fine run's own time to its converged C_D
     if rl["within2_wall"] is not None and rf["converged_wall"]:
         out["speedup_learned_vs_fine"] = rf["converged_wall"] / rl["within2_wall"]

@@ -349,7 +349,7 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
     for (int i = 0; i < Q; i++) in.raw[i] = S::ld_word((const T*)((const char*)src[i] + lane_off));
   } else {
 #pragma unroll
-    for (int i = 0; i < Q; i++) in.raw[i] = src[i][min(x0, xlim[i])];
+    for (int i = 0; i < Q; i++) in.raw[i] = glob(src[i])[min(x0, xlim[i])];
   }
 }
 
