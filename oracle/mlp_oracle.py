@@ -138,10 +138,11 @@ def wall_distance(flags, dims, radius=4):
 
 def coarse_samples(u_fine, solid_fine, dims_fine, wd_coarse, nu0):
     """Training samples from a fine velocity field (n_f, 3): 2x box filter, subgrid stress
-    tau = <u u> - <u><u>, filtered strain by central differences on the coarse grid, target
-    nu_t = max(0, sum(-tau_dev : S) / sum(2 S : S)) over the 3x3x3 neighbourhood (cells without a
-    strain contribute 0). A coarse cell is used when its 8 fine cells and the 8 fine cells of each
-    of its 6 neighbours are fluid. Returns (features (m, 8), target (m,), index (m,))."""
+    tau = <u u> - <u><u>, filtered strain by central differences on the coarse grid (one-sided along
+    an axis with one solid-holding neighbour block), target nu_t = max(0, sum(-tau_dev : S) /
+    sum(2 S : S)) over the 3x3x3 neighbourhood (cells without a strain contribute 0). A coarse cell
+    is used when its 8 fine cells are fluid, it is off the domain faces and along every axis at
+    least one neighbour block is all fluid. Returns (features (m, 8), target (m,), index (m,))."""
     nx, ny, nz = dims_fine
     cx, cy, cz = nx // 2, ny // 2, nz // 2
     u = np.asarray(u_fine, dtype=np.float64).reshape(nz, ny, nx, 3)
@@ -157,8 +158,13 @@ def coarse_samples(u_fine, solid_fine, dims_fine, wd_coarse, nu0):
     g = np.zeros((cz, cy, cx, 3, 3))  # g[..., i, j] = d u_j / d x_i
     valid = ~bad
     for ax, i in ((2, 0), (1, 1), (0, 2)):
-        g[..., i, :] = (np.roll(ub, -1, axis=ax) - np.roll(ub, 1, axis=ax)) / 2.0
-        valid &= ~np.roll(bad, -1, axis=ax) & ~np.roll(bad, 1, axis=ax)
+        # central difference; one-sided where one neighbour block holds a solid cell
+        up, dn = ~np.roll(bad, -1, axis=ax), ~np.roll(bad, 1, axis=ax)
+        u_up = np.where(up[..., None], np.roll(ub, -1, axis=ax), ub)
+        u_dn = np.where(dn[..., None], np.roll(ub, 1, axis=ax), ub)
+        h = np.where(up & dn, 0.5, 1.0)[..., None]
+        g[..., i, :] = h * (u_up - u_dn)
+        valid &= up | dn
     valid[:, :, 0] = valid[:, :, -1] = False
     valid[:, 0, :] = valid[:, -1, :] = False
     valid[0, :, :] = valid[-1, :, :] = False

@@ -85,6 +85,33 @@ def test_coarse_samples_linear_shear():
     assert np.allclose(feats, ref)
 
 
+
+def test_coarse_samples_one_sided_beside_solid():
+    # The same shear with two coarse blocks holding a solid cell, at coarse (1, 2, 1) and (1, 4, 1):
+    # the cells above and below them take one-sided differences, exact for a linear field, so their
+    # strain is the same a / 2; cell (1, 3, 1) has both y neighbours bad and is left out.
+    a = 1e-3
+    dims = (8, 16, 8)
+    z, y, x = np.meshgrid(np.arange(8), np.arange(16), np.arange(8), indexing="ij")
+    u = np.stack([a * (y + 0.5), 0 * y, 0 * y], axis=-1).reshape(-1, 3)
+    solid = np.zeros((8, 16, 8), bool)
+    solid[2, 4, 2] = True  # fine cell of coarse block (1, 2, 1)
+    solid[3, 9, 3] = True  # fine cell of coarse block (1, 4, 1)
+    u[solid.reshape(-1)] = 0.0  # solid cells carry no velocity: a central difference across them is wrong
+    wd = np.full(4 * 8 * 4, 4.0)
+    feats, tgt, idx = M.coarse_samples(u, solid.reshape(-1), dims, wd, 6e-4)
+    cx, cy = 4, 8
+    cells = set((int(i) % cx, int(i) // cx % cy, int(i) // (cx * cy)) for i in idx)
+    interior = set((x, y, z) for x in (1, 2) for y in range(1, 7) for z in (1, 2))
+    assert cells == interior - {(1, 2, 1), (1, 4, 1), (1, 3, 1)}
+    assert (1, 1, 1) in cells and (1, 5, 1) in cells  # one-sided along y
+    assert np.allclose(tgt, 0.0)
+    S = np.zeros((1, 6))
+    S[0, 3] = a
+    ub = a * (2 * (idx // cx % cy) + 1.0)
+    ref = M.raw_features(np.repeat(S, len(idx), 0), np.full(len(idx), 4.0), ub, 6e-4)
+    assert np.allclose(feats, ref)
+
 def test_bf16_forward_close_to_fp64():
     rng = np.random.default_rng(6)
     p = rng.normal(size=M.NPARAM) * 0.3

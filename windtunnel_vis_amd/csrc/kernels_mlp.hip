@@ -143,21 +143,28 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_fields(const typename S::T* __
   bad[c] = b ? 1 : 0;
 }
 
-// Filtered strain of coarse cell c (central differences of the block means); false on the domain
-// faces and next to a bad block.
+// Filtered strain of coarse cell c from the block means: central differences, one-sided along an
+// axis where one neighbour block holds a solid cell (so cells beside the body get a strain and a
+// training target too); false on the domain faces, for a bad block, and where both neighbours
+// along an axis are bad.
 __device__ inline bool coarse_strain(const float* __restrict__ cf, const uint8_t* __restrict__ bad, int cnx, int cny, int cnz, int64_t c,
                                      float (&S)[6]) {
   int x, y, z;
   cell_coords(c, cnx, cny, x, y, z);
   if (x < 1 || y < 1 || z < 1 || x >= cnx - 1 || y >= cny - 1 || z >= cnz - 1) return false;
+  if (bad[c]) return false;
   const int64_t sx = 1, sy = cnx, sz = (int64_t)cnx * cny;
-  if (bad[c] | bad[c + sx] | bad[c - sx] | bad[c + sy] | bad[c - sy] | bad[c + sz] | bad[c - sz]) return false;
   float g[3][3];  // g[i][j] = d u_j / d x_i
   const int64_t st[3] = {sx, sy, sz};
 #pragma unroll
-  for (int i = 0; i < 3; i++)
+  for (int i = 0; i < 3; i++) {
+    const bool up = !bad[c + st[i]], dn = !bad[c - st[i]];
+    if (!up && !dn) return false;
+    const int64_t a = up ? c + st[i] : c, b = dn ? c - st[i] : c;
+    const float h = (up && dn) ? 0.5f : 1.0f;
 #pragma unroll
-    for (int j = 0; j < 3; j++) g[i][j] = 0.5f * (cf[(c + st[i]) * 9 + j] - cf[(c - st[i]) * 9 + j]);
+    for (int j = 0; j < 3; j++) g[i][j] = h * (cf[a * 9 + j] - cf[b * 9 + j]);
+  }
   S[0] = g[0][0]; S[1] = g[1][1]; S[2] = g[2][2];
   S[3] = 0.5f * (g[0][1] + g[1][0]); S[4] = 0.5f * (g[0][2] + g[2][0]); S[5] = 0.5f * (g[1][2] + g[2][1]);
   return true;
