@@ -69,13 +69,20 @@ def run_gpu(dims, steps, warmup, precision="f16", chunk=None):
     n = s.n
     s.step(warmup)
     s.sync()
-    s.profileEnable(True)
+    # events around the timed launches only: events bound to each launch would add queue time
+    # between the launches (a few microseconds each) to the very region being timed
+    s.profileEnable("span")
     s.sync()
     t0 = time.perf_counter()
     s.step(steps)
     s.sync()
     t1 = time.perf_counter()
     k_ms, launches, _ = s.profileRead()
+    s.profileEnable(False)
+    # per-launch event durations over a short second stretch, as a cross-check of the span average
+    s.profileEnable("launch")
+    s.step(min(steps, 200))
+    k_ms_launch, launches_launch, _ = s.profileRead()
     s.profileEnable(False)
     st = s.stepAndSample(0)
     bulk_cells, wall_cells = s.workSplit()
@@ -86,6 +93,7 @@ def run_gpu(dims, steps, warmup, precision="f16", chunk=None):
     k_bytes = fluid * 2 * q * sbytes
     achieved = k_bytes / k_avg_s / 1e9
     return dict(solver=s, body=body, info=info, mlups=mlups, wall=wall, k_avg_ms=k_avg_s * 1e3, fluid=fluid,
+                k_avg_ms_launch_events=k_ms_launch / max(launches_launch, 1),
                 bulk_cells=bulk_cells, wall_cells=wall_cells, k_bytes=k_bytes, achieved_gbs=achieved,
                 unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes, tune=tune)
 
@@ -102,7 +110,7 @@ def precision_stress(dims=(1024, 512, 512), steps=20, warmup=5):
         info = s.voxelizeSdf(body.prims())
         tune = tuning.autotune(s, steps=10)
         s.step(warmup)
-        s.profileEnable(True)
+        s.profileEnable("span")
         s.sync()
         t0 = time.perf_counter()
         s.step(steps)
@@ -306,6 +314,8 @@ def main():
                        "traffic": traffic_from_profiles("pmc_step_d3q19_f16_%dx%dx%d.txt" % dims),
                        "kernel": "k_step<D3Q19,SF16,2,regularized>", "bytes_per_launch": r["k_bytes"],
                        "avg_launch_ms": round(r["k_avg_ms"], 5), "fluid_cells_per_launch": r["fluid"],
+                       "timing": "HIP event pair on the solver's stream around the timed launches: busy span / launches",
+                       "avg_launch_ms_events_per_launch": round(r["k_avg_ms_launch_events"], 5),
                        "measured_copy_gbs": round(copy_gbs, 1)}
     out["step_roofline_frac"] = round(r["mlups"] * 1e6 * 2 * r["q"] * r["sbytes"] / (HBM_PEAK_GBS * 1e9), 4)
     out["work_split"] = {"bulk_fluid_cells": r["bulk_cells"], "wall_cells": r["wall_cells"]}

@@ -264,10 +264,15 @@ int wt_particles_read(wt_particles* p, float* state4, float* hist4, int32_t* hea
 
 /* ---- profiling -------------------------------------------------------------------------- */
 
-/* When enabled, every step records HIP events around its step kernel (one launch per step: wall
- * cells and bulk cells) on the handle's stream. wt_profile_read synchronises and returns the summed
- * kernel time (ms), the number of launches timed, and 0 in the third slot (kept for hosts that read
- * it); then clears the accumulators. */
+/* Step-kernel timing with HIP events on the handle's stream (one launch per step: wall cells and
+ * bulk cells). WT_PROFILE_LAUNCH binds a start / stop event pair to every launch
+ * (hipExtLaunchKernelGGL): exact per-launch durations, but each bound pair holds the queue for a
+ * few microseconds between launches. WT_PROFILE_SPAN records one pair around the launches of each
+ * wt_step call: the summed time is the stream's busy span, i.e. the launches plus the (sub-
+ * microsecond) gaps between back-to-back launches, with nothing added to the queue between them.
+ * wt_profile_read synchronises and returns the summed time (ms), the number of launches it covers,
+ * and 0 in the third slot (kept for hosts that read it); then clears the accumulators. */
+enum { WT_PROFILE_OFF = 0, WT_PROFILE_LAUNCH = 1, WT_PROFILE_SPAN = 2 };
 int wt_profile_enable(wt_solver* s, int32_t on);
 int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* boundary_ms);
 

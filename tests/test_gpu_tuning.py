@@ -74,3 +74,27 @@ def test_autotune_restores_state_and_applies_choice(tmp_path):
     r2 = tuning.autotune(g, steps=5, cache=str(tmp_path / "t.json"))
     assert r2["cached"] and r2["chunk"] == r["chunk"]
     g.destroy()
+
+
+def test_profile_modes_count_launches_and_leave_results_alone():
+    g, f0 = _solver("d3_full", "f16")
+    ref, _ = _solver("d3_full", "f16")
+    ref.step(9)
+    g.profileEnable("launch")
+    g.step(3)
+    ms_l, n_l, _ = g.profileRead()
+    g.profileEnable("span")
+    g.step(4)
+    g.step(0)
+    g.step(2)
+    ms_s, n_s, _ = g.profileRead()
+    g.profileEnable(False)
+    assert (n_l, n_s) == (3, 6) and ms_l > 0 and ms_s > 0
+    ms_0, n_0, _ = g.profileRead()
+    assert (ms_0, n_0) == (0.0, 0)
+    # timing changes no result: 9 profiled steps equal 9 plain ones
+    assert np.array_equal(g.readPopulations().view(np.uint32), ref.readPopulations().view(np.uint32))
+    with pytest.raises(ValueError):
+        g.profileEnable("sometimes")
+    g.destroy()
+    ref.destroy()

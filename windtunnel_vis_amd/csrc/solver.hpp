@@ -56,8 +56,9 @@ struct wt_solver {
   int64_t step = 0;
   int64_t bytes = 0;
   // profiling
-  bool prof = false;
-  std::vector<hipEvent_t> ev;   // per step: start, stop of the step kernel
+  int prof = 0;                 // WT_PROFILE_* mode
+  std::vector<hipEvent_t> ev;   // event pairs: per launch (WT_PROFILE_LAUNCH) or per step call (WT_PROFILE_SPAN)
+  std::vector<int64_t> ev_launches;  // launches covered by each pair
   int item_chunk = 0;           // WT_TUNE_ITEM_CHUNK (0: one contiguous eighth per XCD)
   int cpt = 2;                  // WT_TUNE_CELLS_PER_THREAD (2 needs an even nx)
   size_t ev_used = 0;

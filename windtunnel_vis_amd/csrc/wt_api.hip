@@ -131,23 +131,36 @@ int do_steps(wt_solver* s, int steps) {
   }
   LaunchCtx c = s->ctx();
   const StepArgs A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->item_chunk);
+  // next free event pair, covering `launches` step launches
+  auto pair = [&](int64_t launches, hipEvent_t*& e) -> int {
+    if (s->ev_used + 2 > s->ev.size()) {
+      const size_t old = s->ev.size();
+      s->ev.resize(std::max<size_t>(64, old * 2));
+      for (size_t i = old; i < s->ev.size(); i++) HIP_TRY(hipEventCreate(&s->ev[i]));
+    }
+    e = &s->ev[s->ev_used];
+    s->ev_used += 2;
+    s->ev_launches.push_back(launches);
+    return WT_OK;
+  };
+  hipEvent_t* span = nullptr;
+  if (s->prof == WT_PROFILE_SPAN && steps > 0) {
+    if (int r = pair(steps, span)) return r;
+    HIP_TRY(hipEventRecord(span[0], s->stream));
+  }
   for (int k = 0; k < steps; k++) {
     const void* fin = s->f[s->parity];
     void* fout = s->f[s->parity ^ 1];
-    if (s->prof) {
-      if (s->ev_used + 2 > s->ev.size()) {
-        const size_t old = s->ev.size();
-        s->ev.resize(std::max<size_t>(64, old * 2));
-        for (size_t i = old; i < s->ev.size(); i++) HIP_TRY(hipEventCreate(&s->ev[i]));
-      }
-      hipEvent_t* e = &s->ev[s->ev_used];
-      s->ev_used += 2;
+    if (s->prof == WT_PROFILE_LAUNCH) {
+      hipEvent_t* e = nullptr;
+      if (int r = pair(1, e)) return r;
       launch_step(c, fin, fout, A, e[0], e[1]);
     } else {
       launch_step(c, fin, fout, A);
     }
     s->parity ^= 1;
   }
+  if (span) HIP_TRY(hipEventRecord(span[1], s->stream));
   HIP_TRY(hipGetLastError());
   s->step += steps;
   return WT_OK;
@@ -739,8 +752,10 @@ int wt_particles_read(wt_particles* p, float* state4, float* hist4, int32_t* hea
 int wt_profile_enable(wt_solver* s, int32_t on) {
   if (!s) return fail(WT_E_ARG, "null argument");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  s->prof = on != 0;
+  if (on < WT_PROFILE_OFF || on > WT_PROFILE_SPAN) return fail(WT_E_ARG, "profile mode must be 0, 1 or 2");
+  s->prof = on;
   s->ev_used = 0;
+  s->ev_launches.clear();
   s->acc_bulk = s->acc_bnd = 0;
   s->acc_launches = 0;
   return WT_OK;
@@ -753,9 +768,10 @@ int wt_profile_read(wt_solver* s, double* bulk_ms, int64_t* launches, double* bo
     float a = 0;
     HIP_TRY(hipEventElapsedTime(&a, s->ev[i], s->ev[i + 1]));
     s->acc_bulk += a;
-    s->acc_launches++;
+    s->acc_launches += s->ev_launches[i / 2];
   }
   s->ev_used = 0;
+  s->ev_launches.clear();
   if (bulk_ms) *bulk_ms = s->acc_bulk;
   if (launches) *launches = s->acc_launches;
   if (boundary_ms) *boundary_ms = s->acc_bnd;
@@ -840,7 +856,7 @@ int wt_autotune(wt_solver* s, int32_t steps, wt_tune_result* out) {
   if (hipMalloc(&keep, fbytes) != hipSuccess) return fail(WT_E_NOMEM, "autotune snapshot allocation failed");
   const int parity0 = s->parity;
   const int64_t step0 = s->step;
-  const bool prof0 = s->prof;
+  const int prof0 = s->prof;
   const int chunk0 = s->item_chunk;
   int rc = WT_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -848,7 +864,7 @@ int wt_autotune(wt_solver* s, int32_t steps, wt_tune_result* out) {
     HIP_TRY(hipMemcpyAsync(keep, s->f[parity0], fbytes, hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    s->prof = false;
+    s->prof = WT_PROFILE_OFF;
     double best = 1e30;
     int n = 0;
     for (int cpt : {2, 1}) {

@@ -278,8 +278,17 @@ class SolverGPU:
         return out
 
     # -- profiling --------------------------------------------------------------------------
+    PROFILE_MODES = {False: 0, True: 1, "off": 0, "launch": 1, "span": 2}
+
     def profileEnable(self, on=True):
-        N.check(self._lib.wt_profile_enable(self._h, 1 if on else 0))
+        """Step-kernel timing: True / "launch" binds HIP events to every launch (exact durations,
+        a few microseconds of queue time added between launches); "span" times each step() call's
+        launches as one stretch of the stream (no events between launches; the per-launch average
+        includes the gaps between back-to-back launches); False turns it off."""
+        mode = self.PROFILE_MODES.get(on) if isinstance(on, (bool, str)) else int(on)
+        if mode is None:
+            raise ValueError("profile mode must be one of %s" % sorted(k for k in self.PROFILE_MODES if isinstance(k, str)))
+        N.check(self._lib.wt_profile_enable(self._h, mode))
 
     def profileRead(self):
         b = N.C.c_double()
