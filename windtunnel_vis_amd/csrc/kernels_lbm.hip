@@ -457,32 +457,34 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
 // inputs of every cell of the wave go through the network on the MFMA path (all 64 lanes take
 // part, so the skip tests come after it), and each cell relaxes with tau = t0 + 3 nu_t,
 // nu_t = max(network, smag_floor x Smagorinsky); a non-finite output keeps Smagorinsky.
+// Register budget (five waves per SIMD without spills): only the moments of the two cells stay
+// live through the network evaluation, whose biases are read from LDS per group of 32 cells;
+// the pulled values do not: the regularized outputs follow from the moments alone, and the BGK
+// rows (ground rows) load their populations a second time after the network (L2 hits: the same
+// wave read them just before). The collision of the row is resolved at compile time after the
+// network only.
 template <class L, class S, int CPT, bool REG>
-__device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout, const KParams& P, const RowPos& r,
-                                                const BulkIn<L, S, CPT>& in, const MlpDev* __restrict__ net, short* feats,
+__device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                                const uint8_t* __restrict__ kind, const KParams& P, const RowPos& r,
+                                                const BulkIn<L, S, CPT>& in0, const MlpDev* __restrict__ net, short* feats,
                                                 float* nus) {
   using T = typename S::T;
+  using In = BulkIn<L, S, CPT>;
   constexpr int Q = L::Q;
   const int NX = P.nx;
   const int x0 = r.x0;
-  const int64_t n = P.n;
   const int64_t rowoff = (int64_t)r.row * Q * NX;  // the row's population block
-  const uint32_t k = in.k;
+  const uint32_t k = in0.k;
   constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
-  auto store = [&](int i, float a, float b) {
-    T* o = (T*)((char*)(fout + rowoff + glob(r.pull)[PULL_CLASSES * Q + i]) + lane_off);
-    if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
-    else S::st_dev(o, 0, L::w(i), a);
-  };
   const bool yLo = r.iy == 0;
   const bool gwall = yLo && P.ground != 0;
-  const bool reg = REG && !gwall;
+  const bool reg_row = REG && !gwall;
   const float U = P.uin;
   const float beltA = (P.ground == 2 && !(k & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
   const float beltB = (P.ground == 2 && !((k >> 8) & KIND_BELT_HOLD)) ? 1.0f : 0.0f;
-  auto pull_a = [&](int i) -> float {
+  auto pull_a = [&](const In& in, int i) -> float {
     const int cx = L::c(i, 0), cy = L::c(i, 1);
     float v;
     if constexpr (CPT == 2) v = S::lo_dev(in.raw[i], L::w(i));
@@ -494,7 +496,7 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
     }
     return v;
   };
-  auto pull_b = [&](int i) -> float {
+  auto pull_b = [&](const In& in, int i) -> float {
     const int cx = L::c(i, 0), cy = L::c(i, 1);
     const bool grnd = cy > 0 && yLo && P.ground != 0;
     const bool clamp = i > 0 && cx < 0 && !grnd && x0 + 2 == NX;
@@ -502,19 +504,18 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
     if (grnd && cx != 0) v += beltB * ((6.0f * L::w(i) * cx) * P.uGround);
     return v;
   };
-  Relax ma, mb;
-  {
+  auto moments_a = [&](const In& in, bool reg) {
     float f[Q];
 #pragma unroll
-    for (int i = 0; i < Q; i++) f[i] = pull_a(i);
-    ma = relax<L>(f, (float)x0, gwall, reg, P);
-  }
-  if constexpr (CPT == 2) {
+    for (int i = 0; i < Q; i++) f[i] = pull_a(in, i);
+    return relax<L>(f, (float)x0, gwall, reg, P);
+  };
+  auto moments_b = [&](const In& in, bool reg) {
     float f[Q];
 #pragma unroll
-    for (int i = 0; i < Q; i++) f[i] = pull_b(i);
-    mb = relax<L>(f, (float)(x0 + 1), gwall, reg, P);
-  }
+    for (int i = 0; i < Q; i++) f[i] = pull_b(in, i);
+    return relax<L>(f, (float)(x0 + 1), gwall, reg, P);
+  };
   // inputs of both cells, normalised, staged as BF16 for the wave's MFMA evaluation
   const int lane = threadIdx.x & 63;
   const bool live = x0 < NX && k != SOLID_ALL && !(k & SKIP);
@@ -525,48 +526,76 @@ __device__ __forceinline__ void bulk_finish_mlp(typename S::T* __restrict__ fout
     float x[MLP_IN];
     mlp_raw_features(sc * m.pxx, sc * m.pyy, sc * m.pzz, sc * m.pxy, sc * m.pxz, sc * m.pyz, (float)wdb * (1.0f / 32.0f),
                      __builtin_sqrtf(m.ux * m.ux + m.uy * m.uy + m.uz * m.uz), inu0, x);
+    float v[MLP_IN];
 #pragma unroll
     for (int i = 0; i < MLP_IN; i++) {
-      const float v = live ? (x[i] - net->norm.mu[i]) * net->norm.isig[i] : 0.0f;
-      feats[slot * MLP_IN + i] = to_bf16(v == v ? v : 0.0f);
+      const float y = live ? (x[i] - net->norm.mu[i]) * net->norm.isig[i] : 0.0f;
+      v[i] = y == y ? y : 0.0f;
     }
+    uint32_t* dst = (uint32_t*)(feats + slot * MLP_IN);
+#pragma unroll
+    for (int i = 0; i < MLP_IN / 2; i++) dst[i] = pack_bf16(v[2 * i], v[2 * i + 1]);
   };
-  stage(ma, in.wd & 0xffu, 2 * lane);
-  if constexpr (CPT == 2) stage(mb, (in.wd >> 8) & 0xffu, 2 * lane + 1);
-  else stage(ma, 0u, 2 * lane + 1);
+  Relax ma = moments_a(in0, reg_row), mb;
+  stage(ma, in0.wd & 0xffu, 2 * lane);
+  if constexpr (CPT == 2) {
+    mb = moments_b(in0, reg_row);
+    stage(mb, (in0.wd >> 8) & 0xffu, 2 * lane + 1);
+  } else {
+    stage(ma, 0u, 2 * lane + 1);
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   mlp_wave_nu(net, feats, nus);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // nu_t = max(network, floor x nu_smag), nu_smag = (tau_smag - t0) / 3; non-finite: Smagorinsky
-  const float floor_ = net->norm.smag_floor;
-  auto closure_tau = [&](Relax& m, float nu) {
-    if (nu == nu && nu < 1e3f) {
-      const float nus_ = (fast_rcp(m.omega) - m.t0) * (1.0f / 3.0f);
-      retau(m, m.t0 + 3.0f * fmaxf(nu, floor_ * nus_), reg);
-    }
+  const float nu_a = nus[2 * lane], nu_b = nus[2 * lane + 1];
+  if (x0 >= NX || (k != SOLID_ALL && (k & SKIP))) return;
+  // opaque copies of the source and pull-table pointers, so that the compiler neither merges the
+  // BGK rows' second loads with the first ones nor keeps the first ones' values live through the
+  // network
+  const T* f2 = fin;
+  RowPos r2 = r;
+  asm volatile("" : "+s"(f2), "+s"(r2.pull));
+  auto store = [&](int i, float a, float b) {
+    T* o = (T*)((char*)(fout + rowoff + glob(r2.pull)[PULL_CLASSES * Q + i]) + lane_off);
+    if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
+    else S::st_dev(o, 0, L::w(i), a);
   };
-  closure_tau(ma, nus[2 * lane]);
-  if constexpr (CPT == 2) closure_tau(mb, nus[2 * lane + 1]);
-  if (x0 >= NX) return;
   if (k == SOLID_ALL) {
 #pragma unroll
     for (int i = 0; i < Q; i++) store(i, 0.0f, 0.0f);
     return;
   }
-  if (k & SKIP) return;
-  store(0, rest_out<L>(ma, reg, pull_a(0), P), CPT == 2 ? rest_out<L>(mb, reg, pull_b(0), P) : 0.0f);
+  // nu_t = max(network, floor x nu_smag), nu_smag = (tau_smag - t0) / 3; non-finite: Smagorinsky
+  const float floor_ = net->norm.smag_floor;
+  auto finish = [&](auto rg) {
+    constexpr bool reg = decltype(rg)::value;
+    In in;
+    if constexpr (!reg) bulk_load<L, S, CPT>(f2, kind, P, r2, in);
+    else in = in0;  // unused: the regularized outputs follow from the moments
+    auto closure_tau = [&](Relax& m, float nu) {
+      if (nu == nu && nu < 1e3f) {
+        const float nus_ = (fast_rcp(m.omega) - m.t0) * (1.0f / 3.0f);
+        retau(m, m.t0 + 3.0f * fmaxf(nu, floor_ * nus_), reg);
+      }
+    };
+    closure_tau(ma, nu_a);
+    if constexpr (CPT == 2) closure_tau(mb, nu_b);
+    store(0, rest_out<L>(ma, reg, pull_a(in, 0), P), CPT == 2 ? rest_out<L>(mb, reg, pull_b(in, 0), P) : 0.0f);
 #pragma unroll
-  for (int i = 1; i < Q; i++) {
-    const int o = opp<L>(i);
-    if (o < i) continue;
-    float ai, ao, bi = 0.0f, bo = 0.0f;
-    pair_out<L>(ma, reg, i, pull_a(i), pull_a(o), ai, ao, P);
-    if constexpr (CPT == 2) pair_out<L>(mb, reg, i, pull_b(i), pull_b(o), bi, bo, P);
-    store(i, ai, bi);
-    store(o, ao, bo);
-  }
+    for (int i = 1; i < Q; i++) {
+      const int o = opp<L>(i);
+      if (o < i) continue;
+      float ai, ao, bi = 0.0f, bo = 0.0f;
+      pair_out<L>(ma, reg, i, pull_a(in, i), pull_a(in, o), ai, ao, P);
+      if constexpr (CPT == 2) pair_out<L>(mb, reg, i, pull_b(in, i), pull_b(in, o), bi, bo, P);
+      store(i, ai, bi);
+      store(o, ao, bo);
+    }
+  };
+  if (reg_row) finish(std::true_type{});
+  else finish(std::false_type{});
 }
 
 // One 32-byte wall link, read as two 16-byte global loads.
@@ -634,7 +663,7 @@ __device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin
   const RowPos r = item_pos(item, A, P, CPT);
   bulk_load<L, S, CPT, MLP>(fin, kind, P, r, in, wdist);
   if constexpr (MLP) {
-    bulk_finish_mlp<L, S, CPT, REG>(fout, P, r, in, net, feats, nus);
+    bulk_finish_mlp<L, S, CPT, REG>(fin, fout, kind, P, r, in, net, feats, nus);
   } else {
     // ground rows relax with BGK (wall cells, wgslLbm.ts:186); the branch is wave-uniform
     if (REG && !(r.iy == 0 && P.ground != 0)) bulk_finish<L, S, CPT, true>(fout, P, r, in);
@@ -655,7 +684,7 @@ __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename 
 
 // The step with the learned closure in the bulk (D3Q19); wall cells keep Smagorinsky.
 template <class S, int CPT, bool REG>
-__global__ __launch_bounds__(BLOCK, 3) void k_step_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+__global__ __launch_bounds__(BLOCK, 5) void k_step_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                                                     const uint8_t* __restrict__ kind, const uint8_t* __restrict__ wdist,
                                                     const MlpDev* __restrict__ net, KParams P, StepArgs A) {
   __shared__ short feats[BLOCK / 64][128 * MLP_IN];

@@ -34,11 +34,12 @@ struct MlpNorm {
   float pad[2];
 };
 
-// log(1 + y) for y >= 0: the hardware logarithm on the device (the inputs only need a few
-// significant digits: they are normalised and rounded to BF16 before the network)
+// log(1 + y) for y >= 0: the hardware base-2 logarithm on the device, which needs no range
+// handling here (its argument is >= 1, never denormal); the inputs only need a few significant
+// digits: they are normalised and rounded to BF16 before the network
 __host__ __device__ inline float log1p_fast(float y) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return __logf(1.0f + y);
+  return __builtin_amdgcn_logf(1.0f + y) * 0.69314718f;
 #else
   return log1pf(y);
 #endif
@@ -60,7 +61,11 @@ __host__ __device__ inline void mlp_raw_features(float sxx, float syy, float szz
   const float i2 = 0.5f * (i1 * i1 - tr2);
   const float i3 = sxx * (syy * szz - syz * syz) - sxy * (sxy * szz - syz * sxz) + sxz * (sxy * syz - syy * sxz);
   const float tr3 = i1 * i1 * i1 - 3.0f * i1 * i2 + 3.0f * i3;  // Cayley-Hamilton
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float m = __builtin_amdgcn_sqrtf(2.0f * tr2);  // hardware square root (about 1 ulp)
+#else
   const float m = sqrtf(2.0f * tr2);
+#endif
   constexpr float ir = 1.0f / MLP_S_REF, ir2 = ir * ir, ir3 = ir2 * ir;
   x[0] = slog(i1, ir);
   x[1] = slog(i2, ir2);
@@ -74,7 +79,8 @@ __host__ __device__ inline void mlp_raw_features(float sxx, float syy, float szz
 
 __host__ __device__ inline float softplus(float z) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return z > 20.0f ? z : __logf(1.0f + __expf(z));
+  // hardware base-2 exponential and logarithm (1 + e^z >= 1: no range handling needed)
+  return z > 20.0f ? z : __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(z * 1.44269504f)) * 0.69314718f;
 #else
   return z > 20.0f ? z : log1pf(expf(z));
 #endif
@@ -101,6 +107,20 @@ __device__ inline short to_bf16(float x) {  // round to nearest even (finite inp
   return (short)(u >> 16);
 }
 
+// Two floats to packed BF16 (a in the low half) with gfx950's v_cvt_pk_bf16_f32: round to nearest
+// even, the same bits as to_bf16 for every non-NaN input (infinities and overflow included).
+typedef __bf16 bf16x2_hw __attribute__((ext_vector_type(2)));
+typedef float f32x2_hw __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t pack_bf16(float a, float b) {
+  const f32x2_hw v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_hw));
+}
+
+// max(x, 0) as one integer max on the bit pattern: negative floats (sign bit set) are negative
+// integers. Gives +0 for -0 and keeps NaNs with a clear sign bit (fmaxf would cost a NaN-quieting
+// max per operand besides).
+__device__ inline float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
 // Device-side network in the layout the MFMA path reads (prepared once from the FP32 master).
 struct MlpDev {
   bf16x4_t w1a[64];     // A operand of layer 1 per lane
@@ -124,26 +144,33 @@ __device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* 
   bf16x4_t w2[4];
 #pragma unroll
   for (int c = 0; c < 4; c++) w2[c] = net->w2a[c][l];
-#pragma unroll
+  // one group of 32 cells at a time, the biases and output weights read from LDS in each pass:
+  // kept in registers across the groups (48 of them, plus interleaved accumulators) they would
+  // set the register budget of a kernel that fuses this evaluation
+#pragma unroll 1
   for (int g = 0; g < 4; g++) {
     const int cell = 32 * g + (l & 31);
     const bf16x4_t xb = *(const bf16x4_t*)(feats + cell * MLP_IN + 4 * h);
+    int hg = h;
+    asm volatile("" : "+v"(hg));  // opaque per pass: the LDS reads below stay inside the loop
     f32x16_t acc;
 #pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = net->b1d[h][r];
+    for (int r = 0; r < 16; r++) acc[r] = net->b1d[hg][r];
     acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1, xb, acc, 0, 0, 0);
     bf16x4_t hb[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++)
+    for (int c = 0; c < 4; c++) {
+      const uint32_t lo = pack_bf16(relu(acc[4 * c]), relu(acc[4 * c + 1]));
+      const uint32_t hi = pack_bf16(relu(acc[4 * c + 2]), relu(acc[4 * c + 3]));
+      hb[c] = __builtin_bit_cast(bf16x4_t, (uint64_t)lo | ((uint64_t)hi << 32));
+    }
 #pragma unroll
-      for (int t = 0; t < 4; t++) hb[c][t] = to_bf16(fmaxf(acc[4 * c + t], 0.0f));
-#pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = net->b2d[h][r];
+    for (int r = 0; r < 16; r++) acc[r] = net->b2d[hg][r];
 #pragma unroll
     for (int c = 0; c < 4; c++) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2[c], hb[c], acc, 0, 0, 0);
     float z = 0.0f;
 #pragma unroll
-    for (int r = 0; r < 16; r++) z = fmaf(net->w3d[h][r], fmaxf(acc[r], 0.0f), z);
+    for (int r = 0; r < 16; r++) z = fmaf(net->w3d[hg][r], relu(acc[r]), z);
     z += __shfl_xor(z, 32);
     if (h == 0) nu[cell] = net->norm.nu_scale * fminf(softplus(z + net->b3), MLP_TARGET_CLIP);
   }
