@@ -98,6 +98,37 @@ def run_gpu(dims, steps, warmup, precision="f16", chunk=None):
                 unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes, tune=tune)
 
 
+def learned_step_bench(dims, steps, warmup, plain_ms, precision="f16"):
+    """The step with the learned closure (k_step_mlp: BF16 MFMA inference fused in the step) on the
+    same sedan domain and launch configuration as the plain line (2 cells per lane, plain item
+    order). The network is an untrained one of the trained network's shape and normalisation: the
+    cost does not depend on the weights. Timed as run_gpu times the plain step."""
+    from windtunnel_vis_amd import closure
+    s, body, info = make_sedan_solver(dims, precision)
+    s.setTuning(0, 4)
+    net = closure.Network.initial(nu_scale=1e-4)
+    net.sigma = np.full(8, 3.0, np.float32)
+    closure.attach(s, net)
+    s.step(warmup)
+    s.profileEnable("span")
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    k_ms, launches, _ = s.profileRead()
+    s.profileEnable(False)
+    st = s.stepAndSample(0)
+    s.destroy()
+    k_avg = k_ms / max(launches, 1)
+    fluid = s.n - info["solidCells"]
+    sb = 4 if precision == "f32" else 2
+    return {"kernel": "k_step_mlp<SF16,2,regularized>", "mlups": round(s.n * steps / wall / 1e6, 1),
+            "avg_launch_ms": round(k_avg, 5), "hbm_frac": round(fluid * 2 * 19 * sb / (k_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "time_vs_plain_step": round(k_avg / plain_ms, 3) if plain_ms else None, "steps": steps,
+            "unstable": bool(st.unstable)}
+
+
 def precision_stress(dims=(1024, 512, 512), steps=20, warmup=5):
     """The sports-wedge setup (300x90x45 cells, corner radius 10, u = 0.1) at 1024x512x512 in each
     storage format: MLUPS and the step kernel's HBM roofline fraction (autotuned launch)."""
@@ -288,6 +319,8 @@ def main():
     ap.add_argument("--no-stress", action="store_true", help="skip the 1024x512x512 precision stress")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: copy probe + the main workload only")
+    ap.add_argument("--no-learned-step", action="store_true", help="skip timing the learned-closure step")
+    ap.add_argument("--learned-step", action="store_true", help="time the learned-closure step also with --profile-run")
     ap.add_argument("--chunk", type=int, default=None,
                     help="fixed item-run length instead of the autotuner (4 = plain order, 0 = eighths)")
     args = ap.parse_args()
@@ -346,6 +379,9 @@ def main():
                         "cycles": w["cycles"], "steps": w["steps"]}
         out["config1_cylinder"] = {"window_20k": rd(c1["window_20k"]), "developed": rd(c1["developed"]),
                                    "unstable": c1["unstable"], "seconds": round(c1["seconds"], 2)}
+    if args.learned_step or not (args.no_learned_step or args.profile_run):
+        out["learned_step"] = learned_step_bench(dims, max(min(args.steps, 2000) // 4, 50), args.warmup, r["k_avg_ms"],
+                                                 args.precision)
     if not args.no_sweep:
         out["batched_sweep"] = sweep_bench()
     if not args.no_closure:
