@@ -98,6 +98,43 @@ def run_gpu(dims, steps, warmup, precision="f16", chunk=None):
                 unstable=st.unstable, maxU=st.maxU, q=q, sbytes=sbytes, tune=tune)
 
 
+def shard_bench(steps, warmup, precision="f16", parts=4):
+    """One GPU's share of the 512x256x256 sedan domain cut into `parts` x-slabs: the slab that holds
+    the body (x in [128, 256) for four parts) as its own 128x256x256 grid, with the cut planes as the
+    domain's inlet / outlet faces (the halo exchange that would replace them is not built). Measures
+    the step kernel at the shard's real shape (rows of 128 cells instead of 512)."""
+    from windtunnel_vis_amd import SolverGPU, tuning
+    from windtunnel_vis_amd import geometry as G
+    full = (512, 256, 256)
+    w = full[0] // parts
+    x0 = w * int(0.34 * full[0] // w)  # the slab holding the body's centre
+    dims = (w, full[1], full[2])
+    s = SolverGPU(dims, workload_params(precision=precision))
+    body = G.sedan(full)
+    body.translate((-x0, 0, 0))
+    s.setWheels(body.wheels)
+    info = s.voxelizeSdf(body.prims())
+    tune = tuning.autotune(s, steps=20)
+    s.step(warmup)
+    s.profileEnable("span")
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    k_ms, launches, _ = s.profileRead()
+    s.profileEnable(False)
+    st = s.stepAndSample(0)
+    s.destroy()
+    k_avg = k_ms / max(launches, 1)
+    fluid = s.n - info["solidCells"]
+    sb = 4 if precision == "f32" else 2
+    return {"slab": "x in [%d, %d) of 512x256x256 (%d x-slabs)" % (x0, x0 + w, parts), "grid": list(dims),
+            "solid_cells": info["solidCells"], "mlups": round(s.n * steps / wall / 1e6, 1), "avg_launch_ms": round(k_avg, 5),
+            "kernel_frac": round(fluid * 2 * 19 * sb / (k_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "chunk": tune["chunk"],
+            "unstable": bool(st.unstable)}
+
+
 def learned_step_bench(dims, steps, warmup, plain_ms, precision="f16"):
     """The step with the learned closure (k_step_mlp: BF16 MFMA inference fused in the step) on the
     same sedan domain and launch configuration as the plain line (2 cells per lane, plain item
@@ -381,6 +418,8 @@ def main():
                         "cycles": w["cycles"], "steps": w["steps"]}
         out["config1_cylinder"] = {"window_20k": rd(c1["window_20k"]), "developed": rd(c1["developed"]),
                                    "unstable": c1["unstable"], "seconds": round(c1["seconds"], 2)}
+    if not (args.no_north_star or args.profile_run):
+        out["shard_128x256x256"] = shard_bench(max(min(args.steps, 2000) // 4, 100), args.warmup, args.precision)
     if args.learned_step or not (args.no_learned_step or args.profile_run):
         out["learned_step"] = learned_step_bench(dims, max(min(args.steps, 2000) // 4, 50), args.warmup, r["k_avg_ms"],
                                                  args.precision)
