@@ -97,6 +97,14 @@ def make_case(name):
         dims = (17, 9, 7)  # odd n: the last cell has no pair partner
         f, s, w = box_body_3d(dims)
         return dims, dict(base3), f, s, w
+    if name == "d3_long_rows":
+        dims = (384, 6, 5)  # three waves per row: the middle one holds neither row end
+        f, s, w = box_body_3d(dims)
+        return dims, dict(base3), f, s, w
+    if name == "d2_long_rows":
+        dims = (300, 10, 1)
+        f, s = cylinder_2d(dims)
+        return dims, dict(base2), f, s, []
     if name == "d2_cyl":
         dims = (40, 20, 1)
         f, s = cylinder_2d(dims)

@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = {"f32": 1e-5, "f16": 2e-3, "bf16": 1e-2}
 ORACLE_STORAGE = {"f32": None, "f16": "f16", "bf16": "bf16"}
-CASES = ["d3_full", "d3_periodic_bgk", "d3_freeslip_halfway", "d3_odd_f32", "d2_cyl", "d2_moving_bgk"]
+CASES = ["d3_full", "d3_periodic_bgk", "d3_freeslip_halfway", "d3_odd_f32", "d2_cyl", "d2_moving_bgk", "d3_long_rows",
+         "d2_long_rows"]
 
 
 def _setup(solver, dims, params, flags, sdf, wheels, f0):
