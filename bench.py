@@ -350,8 +350,8 @@ def main():
     ap.add_argument("--no-physics", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU-baseline sample steps (0 = auto)")
-    ap.add_argument("--north-star", action="store_true", help="also time 512x256x256 (the default; kept for old command lines)")
-    ap.add_argument("--no-north-star", action="store_true", help="skip the 512x256x256 line")
+    ap.add_argument("--large", action="store_true", help="also time 512x256x256 (the default; kept for old command lines)")
+    ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 and slab lines")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched 24-domain sweep")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure study")
     ap.add_argument("--no-stress", action="store_true", help="skip the 1024x512x512 precision stress")
@@ -363,13 +363,13 @@ def main():
                     help="fixed item-run length instead of the autotuner (4 = plain order, 0 = eighths)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     dims = tuple(int(v) for v in args.dims.split(","))
-    args.north_star = not args.no_north_star and dims != (512, 256, 256)
+    args.large = not args.no_large and dims != (512, 256, 256)
     if args.profile_run:
         args.no_physics = args.no_cpu = args.no_sweep = args.no_closure = args.no_stress = True
-        args.north_star = False
+        args.large = False
     copy_gbs = copy_probe(256 << 20, 3) if args.profile_run else copy_probe()
     r = run_gpu(dims, args.steps, args.warmup, args.precision, chunk=args.chunk)
     out = {
@@ -402,23 +402,23 @@ def main():
         out["physics"] = {"cd": round(ph["cd"], 4), "cl": round(ph["cl"], 4), "frontal_cells": ph["frontal"],
                           "unstable": ph["unstable"], "steps": 5000}
     r["solver"].destroy()
-    if args.north_star:
+    if args.large:
         big = (512, 256, 256)
         rb = run_gpu(big, max(args.steps // 4, 200), args.warmup, args.precision)
-        out["north_star_512x256x256"] = {"mlups": round(rb["mlups"], 1), "chunk": rb["tune"]["chunk"],
+        out["d3q19_f16_512x256x256"] = {"mlups": round(rb["mlups"], 1), "chunk": rb["tune"]["chunk"],
                                           "step_roofline_frac": round(rb["mlups"] * 1e6 * 2 * rb["q"] * rb["sbytes"] / (HBM_PEAK_GBS * 1e9), 4),
                                           "kernel_frac": round(rb["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                           "avg_launch_ms": round(rb["k_avg_ms"], 5)}
         rb["solver"].destroy()
     if not args.no_physics:
-        from windtunnel_vis_amd.validation import config1_cylinder
+        from windtunnel_vis_amd.validation import cylinder_512x128
         from windtunnel_vis_amd import SolverGPU
-        c1 = config1_cylinder(lambda d, p: SolverGPU(d, p))
+        c1 = cylinder_512x128(lambda d, p: SolverGPU(d, p))
         rd = lambda w: {"St": round(w["St"], 4) if w["St"] == w["St"] else None, "cd": round(w["cd"], 4),
                         "cycles": w["cycles"], "steps": w["steps"]}
-        out["config1_cylinder"] = {"window_20k": rd(c1["window_20k"]), "developed": rd(c1["developed"]),
+        out["cylinder_512x128"] = {"window_20k": rd(c1["window_20k"]), "developed": rd(c1["developed"]),
                                    "unstable": c1["unstable"], "seconds": round(c1["seconds"], 2)}
-    if not (args.no_north_star or args.profile_run):
+    if not (args.no_large or args.profile_run):
         out["shard_128x256x256"] = shard_bench(max(min(args.steps, 2000) // 4, 100), args.warmup, args.precision)
     if args.learned_step or not (args.no_learned_step or args.profile_run):
         out["learned_step"] = learned_step_bench(dims, max(min(args.steps, 2000) // 4, 50), args.warmup, r["k_avg_ms"],

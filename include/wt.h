@@ -168,7 +168,7 @@ typedef struct {
   float param;         /* wedge: slope of the top face (height lost per unit length), else 0 */
 } wt_prim;
 
-/* Voxelize an analytic body into the solver (no reference counterpart; the configs' procedural
+/* Voxelize an analytic body into the solver (no reference counterpart; the benchmark workloads' procedural
  * bodies). Writes flags (solid iff the cell centre is inside; cells inside a wheel cylinder of the
  * solver's wheel list are tagged as VoxelizerGPU's wheel pass does, wgslVoxel.ts:268-290) and
  * distances |sdf| < 2.5 at cell centres (the roundFlags convention, validation.ts:35-53). */

@@ -8,11 +8,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_config1_cylinder_strouhal_and_drag():
+def test_cylinder_512x128_strouhal_and_drag():
     from windtunnel_vis_amd import SolverGPU
-    from windtunnel_vis_amd.validation import config1_cylinder
-    print("config1 cylinder: 512x128 D2Q9 FP32, 40k steps", flush=True)
-    r = config1_cylinder(lambda d, p: SolverGPU(d, p))
+    from windtunnel_vis_amd.validation import cylinder_512x128
+    print("cylinder 512x128 D2Q9 FP32, 40k steps", flush=True)
+    r = cylinder_512x128(lambda d, p: SolverGPU(d, p))
     print(r, flush=True)
     assert not r["unstable"]
     dev = r["developed"]

@@ -1,8 +1,8 @@
 """GPU vs oracle parity on tiny grids after 100 steps (populations, statistics, macro fields).
 
 Tolerances (max relative population error |f_gpu - f_oracle| / |f_oracle|):
-  FP32 storage vs FP64 oracle: 1e-5   (north-star bound)
-  FP16 deviation storage vs FP64 oracle with FP16 store rounding: 2e-3   (north-star bound)
+  FP32 storage vs FP64 oracle: 1e-5   (the required bound)
+  FP16 deviation storage vs FP64 oracle with FP16 store rounding: 2e-3   (the required bound)
   BF16 deviation storage vs FP64 oracle with BF16 store rounding: 1e-2   (8-bit mantissa)
 """
 import numpy as np

@@ -93,7 +93,7 @@ def sphere(make, scale=1.0):
 CASES = {"freestream": freestream, "cylinder": cylinder, "sphere": sphere}
 
 
-def config1_cylinder(make, steps=40000, sample_every=59, seed=20261203):
+def cylinder_512x128(make, steps=40000, sample_every=59, seed=20261203):
     """Plain D2Q9 BGK cylinder at 512 x 128 (FP32): D = 20 at (128, 64), staircase bounce-back,
     equilibrium inlet u = 0.05, outflow, slip top / bottom, nu = 0.05 * 20 / 100 (Re = 100), cs = 0,
     the cylinder case's sponge (validation.ts:102). Initial state feq(1, (U, delta)) with delta ~
@@ -148,7 +148,7 @@ def config1_cylinder(make, steps=40000, sample_every=59, seed=20261203):
         return dict(St=cf["f"] * D / U, cycles=cf["cycles"], cl_amp=cf["amplitude"],
                     cd=float(np.mean([cd[i] for i in idx])) if idx else math.nan, steps=[a, b])
 
-    out = dict(id="config1_cylinder", frontal=frontal, unstable=unstable, steps=step, seconds=time.time() - t0,
+    out = dict(id="cylinder_512x128", frontal=frontal, unstable=unstable, steps=step, seconds=time.time() - t0,
                window_20k=window(8000, 20000))
     if steps >= 40000:
         out["developed"] = window(20000, 40000)
