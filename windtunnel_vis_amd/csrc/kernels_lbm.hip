@@ -539,9 +539,11 @@ __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict_
     for (int i = 0; i < Q; i++) f[i] = pull_b(in, i);
     return relax<L>(f, (float)(x0 + 1), gwall, reg, P);
   };
-  // inputs of both cells, normalised, staged as BF16 for the wave's MFMA evaluation
+  // inputs of both cells, normalised, staged as BF16 for the wave's MFMA evaluation. Cells whose
+  // result is not used (solid, wall-path or past the row end) stage their inputs unmasked: a cell
+  // is one column of every product, so its inputs reach no other cell's output; a non-finite input
+  // of a live cell gives a non-finite output, which keeps Smagorinsky.
   const int lane = threadIdx.x & 63;
-  const bool live = x0 < NX && k != SOLID_ALL && !(k & SKIP);
   const float inu0 = 3.0f / (P.tau0 - 0.5f);
   auto stage = [&](const Relax& m, uint32_t wdb, int slot) {
     const float irho = 4.5f * fast_rcp(m.rho45);
@@ -551,10 +553,7 @@ __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict_
                      __builtin_sqrtf(m.ux * m.ux + m.uy * m.uy + m.uz * m.uz), inu0, x);
     float v[MLP_IN];
 #pragma unroll
-    for (int i = 0; i < MLP_IN; i++) {
-      const float y = live ? (x[i] - net->norm.mu[i]) * net->norm.isig[i] : 0.0f;
-      v[i] = y == y ? y : 0.0f;
-    }
+    for (int i = 0; i < MLP_IN; i++) v[i] = (x[i] - net->norm.mu[i]) * net->norm.isig[i];
     uint32_t* dst = (uint32_t*)(feats + slot * MLP_IN);
 #pragma unroll
     for (int i = 0; i < MLP_IN / 2; i++) dst[i] = pack_bf16(v[2 * i], v[2 * i + 1]);

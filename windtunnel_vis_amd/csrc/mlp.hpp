@@ -122,7 +122,7 @@ __device__ inline uint32_t pack_bf16(float a, float b) {
 __device__ inline float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
 // Device-side network in the layout the MFMA path reads (prepared once from the FP32 master).
-struct MlpDev {
+struct alignas(16) MlpDev {
   bf16x4_t w1a[64];     // A operand of layer 1 per lane
   bf16x4_t w2a[4][64];  // A operands of layer 2 per K-chunk and lane
   float b1d[2][16];     // layer-1 bias in D layout, per lane half
