@@ -1069,10 +1069,11 @@ void launch_step_batch(hipStream_t st, int q, int prec, int cpt, bool reg, const
   dispatch(q, prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), 0, st, descs, dmap, wall_total, parity); };
-    if (cpt == 2) {
+    // 16-bit words hold two cells: those formats always run two cells per lane
+    if (S::BYTES == 2 || cpt == 2) {
       if (reg) go(k_step_batch<L, S, 2, true>);
       else go(k_step_batch<L, S, 2, false>);
-    } else {
+    } else if constexpr (S::BYTES == 4) {
       if (reg) go(k_step_batch<L, S, 1, true>);
       else go(k_step_batch<L, S, 1, false>);
     }
@@ -1091,10 +1092,10 @@ void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs
     };
     auto pick = [&](auto s) {
       using S = decltype(s);
-      if (cpt == 2) {
+      if (S::BYTES == 2 || cpt == 2) {
         if (reg) go(k_step_mlp<S, 2, true>, s);
         else go(k_step_mlp<S, 2, false>, s);
-      } else {
+      } else if constexpr (S::BYTES == 4) {
         if (reg) go(k_step_mlp<S, 1, true>, s);
         else go(k_step_mlp<S, 1, false>, s);
       }
@@ -1111,10 +1112,10 @@ void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs
       hipExtLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, c.stream, ev0, ev1, 0, (const typename S::T*)fin, (typename S::T*)fout, c.kind,
                             c.P, A);
     };
-    if (cpt == 2) {
+    if (S::BYTES == 2 || cpt == 2) {
       if (reg) go(k_step<L, S, 2, true>);
       else go(k_step<L, S, 2, false>);
-    } else {
+    } else if constexpr (S::BYTES == 4) {
       if (reg) go(k_step<L, S, 1, true>);
       else go(k_step<L, S, 1, false>);
     }
