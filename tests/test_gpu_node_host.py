@@ -37,6 +37,23 @@ def test_node_matches_python_host():
     np.testing.assert_array_equal(np.array(o["ex"]["vort8"], dtype=np.float32), e.vort.reshape(-1)[:8])
     for k, a in (("vel", e.vel), ("mean", e.mean), ("vort", e.vort)):
         assert abs(o["ex"][k] - float(a.astype(np.float64).sum())) <= 1e-9 * max(1.0, float(np.abs(a).sum()))
+    # tracers through the Node host equal the Python host's (the same kernels on the same state)
+    from windtunnel_vis_amd import Tracers
+    t = o["tracers"]
+    vel, mean = s.probe(np.array([[2.5, 6.0, 5.0], [20.25, 3.5, 7.75], [11.0, 0.2, 9.9]]))
+    np.testing.assert_array_equal(np.array(t["vel"], np.float32), vel.reshape(-1))
+    np.testing.assert_array_equal(np.array(t["mean"], np.float32), mean.reshape(-1))
+    sl = s.streamlines(dict(a=(1.5, 2.0, 3.0), b=(1.5, 10.0, 3.0), d=(0.0, 0.0, 2.0)), 6, 12, 0.5, perLine=3)
+    np.testing.assert_array_equal(np.array(t["sl"], np.float32), sl.reshape(-1))
+    assert (sl[:, 0, 3] > 0).all()  # seeds start in the flow
+    tr = Tracers(s, 64, trail=4, emitter="inlet", maxAge=50)
+    tr.update(0, advance=False, fill=True, seed=1)
+    tr.update(2, seed=2)
+    tr.update(2, seed=3)
+    state, hist, head = tr.read()
+    tr.destroy()
+    np.testing.assert_array_equal(np.array(t["state"], np.float32), state.reshape(-1))
+    assert t["head"] == head and abs(t["trail"] - float(hist.astype(np.float64).sum())) <= 1e-9 * max(1.0, float(np.abs(hist).sum()))
 
 
 def test_node_harness_freestream_and_cylinder():

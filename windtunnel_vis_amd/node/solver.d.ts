@@ -79,8 +79,34 @@ export declare class SolverGPU implements LbmSolver {
   voxelizeMesh(mesh: { positions: ArrayLike<number>; indices: ArrayLike<number> }, matrix: ArrayLike<number>, wheels?: WheelDef[]): VoxelInfo;
   derive(): void;
   exportFields(stride: number): Promise<FieldExport>;
+  /** Velocity and mean records at points (x, y, z triples in cells), 4 floats per point each. */
+  probe(points: ArrayLike<number>): { vel: Float32Array; mean: Float32Array };
+  /** seeds x points x (x, y, z, speed or -1). */
+  streamlines(rake: Rake, seeds: number, points: number, h?: number, perLine?: number, useMean?: boolean): Float32Array;
   destroy(): void;
   static bytesFor(dims: GridDims): number;
 }
 
 export declare const solverFactory: SolverFactory;
+
+/** Seeding line a -> b; parallel lines offset by d (cells). */
+export interface Rake { a: number[]; b: number[]; d?: number[]; }
+
+export interface TracerOptions {
+  count: number;
+  trail?: number;
+  emitter?: 'inlet' | 'rake';
+  nozzles?: number;
+  lines?: number;
+  maxAge?: number;
+  rake?: Rake;
+}
+
+export declare class Tracers {
+  constructor(solver: SolverGPU, opts: TracerOptions);
+  readonly count: number;
+  readonly trail: number;
+  update(steps: number, opts?: { advance?: boolean; fill?: boolean; seed?: number }): void;
+  read(): { state: Float32Array; trail: Float32Array; head: number };
+  destroy(): void;
+}

@@ -127,6 +127,33 @@ export class SolverGPU {
     return { dims: r.dims, stride, vel: r.vel, mean: r.mean, vort: r.vort };
   }
 
+  /**
+   * PROBES_WGSL (render/tracers.ts:109-125): the velocity record and the mean record at points in
+   * cell units (a flat array of x, y, z triples); returns { vel, mean }, 4 floats per point
+   * (ux, uy, uz, rho) sampled linearly with clamp to edge.
+   */
+  probe(points) {
+    const k = points.length / 3;
+    const pos = new Float32Array(4 * k);
+    for (let i = 0; i < k; i++) for (let a = 0; a < 3; a++) pos[4 * i + a] = points[3 * i + a];
+    const out = native.probe(this.h, pos);
+    const vel = new Float32Array(4 * k), mean = new Float32Array(4 * k);
+    for (let i = 0; i < k; i++) for (let a = 0; a < 4; a++) {
+      vel[4 * i + a] = out[8 * i + a];
+      mean[4 * i + a] = out[8 * i + 4 + a];
+    }
+    return { vel, mean };
+  }
+
+  /**
+   * STREAMLINES_WGSL (render/tracers.ts:152-200): `seeds` lines seeded on the rake
+   * ({a, b, d}: 3-vectors; perLine seeds per rake line), `points` points each, step h along the
+   * normalised velocity (midpoint rule); Float32Array of seeds x points x (x, y, z, speed or -1).
+   */
+  streamlines(rake, seeds, points, h = 0.5, perLine = seeds, useMean = false) {
+    return native.streamlines(this.h, rakeArray(rake), seeds, perLine, points, h, !!useMean);
+  }
+
   destroy() {
     if (this.h) native.destroy(this.h);
     this.h = null;
@@ -137,3 +164,44 @@ export class SolverGPU {
 
 /** SolverFactory (src/backend/types.ts:18). */
 export const solverFactory = (dims, p) => new SolverGPU(dims, p);
+
+/** A rake {a, b, d} (3-vectors, d = offset between parallel lines) as the addon's Float32Array(12). */
+function rakeArray(r = {}) {
+  const out = new Float32Array(12);
+  const put = (v, o) => { if (v) for (let k = 0; k < 3; k++) out[o + k] = v[k] || 0; };
+  put(r.a, 0);
+  put(r.b, 4);
+  put(r.d, 8);
+  return out;
+}
+
+/**
+ * PARTICLES_WGSL (render/tracers.ts:3-106): particle tracers advected on a solver's velocity field
+ * with a trail ring buffer. emitter 'inlet' spawns on the inlet plane (x in [1, 3)), 'rake' from
+ * `nozzles` nozzles on each of `lines` rake lines. Destroy it before its solver.
+ */
+export class Tracers {
+  constructor(solver, { count, trail = 16, emitter = 'inlet', nozzles = 16, lines = 1, maxAge = 400, rake } = {}) {
+    this.solver = solver;
+    this.count = count;
+    this.trail = trail;
+    this.frameSeed = 0;
+    this.h = native.particlesCreate(solver.h, {
+      count, trail, emitter: emitter === 'inlet' ? 0 : 1, nozzles, lines, maxAge, rake: rakeArray(rake),
+    });
+  }
+
+  /** One frame (RendererGPU.ts:505-527): `steps` lattice steps of advection; fill respawns all. */
+  update(steps, { advance = true, fill = false, seed } = {}) {
+    if (seed === undefined) seed = this.frameSeed++;
+    native.particlesUpdate(this.h, steps, seed >>> 0, !!fill, !!advance);
+  }
+
+  /** { state: count x (x, y, z, age), trail: trail x count x 4 (slot-major), head }. */
+  read() { return native.particlesRead(this.h); }
+
+  destroy() {
+    if (this.h) native.particlesDestroy(this.h);
+    this.h = null;
+  }
+}

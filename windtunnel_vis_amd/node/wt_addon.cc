@@ -526,6 +526,204 @@ napi_value ExportFields(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// probe(h, Float32Array pos4) -> Float32Array (8 per point: field ux, uy, uz, rho, then the mean)
+napi_value Probe(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  float* pos;
+  size_t np;
+  if (!typed_array(env, argv[1], napi_float32_array, &pos, &np) || np % 4) {
+    napi_throw_type_error(env, nullptr, "probe(h, Float32Array of 4 floats per point)");
+    return nullptr;
+  }
+  void* d;
+  napi_value out = make_typed(env, napi_float32_array, 2 * np, 4, &d);
+  if (!out) {
+    napi_throw_error(env, nullptr, "out of memory");
+    return nullptr;
+  }
+  int rc = wt_probe(h->s, pos, (int32_t)(np / 4), (float*)d);
+  if (rc) return throw_wt(env, rc);
+  return out;
+}
+
+bool get_rake(napi_env env, napi_value v, wt_rake* r) {
+  float* e;
+  size_t n;
+  if (!typed_array(env, v, napi_float32_array, &e, &n) || n != 12) return false;
+  for (int k = 0; k < 4; k++) {
+    r->a[k] = e[k];
+    r->b[k] = e[4 + k];
+    r->d[k] = e[8 + k];
+  }
+  return true;
+}
+
+// streamlines(h, rake Float32Array(12) = a, b, d (xyzw each), seeds, perLine, points, step, useMean)
+//   -> Float32Array (seeds x points x 4: x, y, z, speed or -1)
+napi_value Streamlines(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  if (!args(env, info, 7, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  wt_rake r{};
+  if (!get_rake(env, argv[1], &r)) {
+    napi_throw_type_error(env, nullptr, "streamlines: rake must be a Float32Array(12): a, b, d as xyzw");
+    return nullptr;
+  }
+  int32_t seeds = 0, per = 0, points = 0;
+  double step = 0.5;
+  bool mean = false;
+  napi_get_value_int32(env, argv[2], &seeds);
+  napi_get_value_int32(env, argv[3], &per);
+  napi_get_value_int32(env, argv[4], &points);
+  napi_get_value_double(env, argv[5], &step);
+  napi_get_value_bool(env, argv[6], &mean);
+  if (seeds < 1 || points < 1) {
+    napi_throw_range_error(env, nullptr, "streamlines: seeds and points must be >= 1");
+    return nullptr;
+  }
+  void* d;
+  napi_value out = make_typed(env, napi_float32_array, (size_t)seeds * points * 4, 4, &d);
+  if (!out) {
+    napi_throw_error(env, nullptr, "out of memory");
+    return nullptr;
+  }
+  int rc = wt_streamlines(h->s, &r, seeds, per > 0 ? per : seeds, points, (float)step, mean ? 1 : 0, (float*)d);
+  if (rc) return throw_wt(env, rc);
+  return out;
+}
+
+// A particle system keeps its solver's external alive (a reference) and must be destroyed before
+// the solver is; after the solver's own destroy() it refuses to update or read.
+struct PHandle {
+  wt_particles* p = nullptr;
+  Handle* owner = nullptr;
+  napi_ref owner_ref = nullptr;
+  int32_t count = 0, trail = 0;
+};
+
+void finalize_particles(napi_env env, void* data, void*) {
+  auto* ph = static_cast<PHandle*>(data);
+  if (ph->p) wt_particles_destroy(ph->p);  // touches no solver memory
+  if (ph->owner_ref) napi_delete_reference(env, ph->owner_ref);
+  delete ph;
+}
+
+PHandle* get_particles(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "expected a particle handle");
+    return nullptr;
+  }
+  auto* ph = static_cast<PHandle*>(p);
+  if (!ph->p) {
+    napi_throw_error(env, nullptr, "particle system was destroyed");
+    return nullptr;
+  }
+  if (!ph->owner || !ph->owner->s) {
+    napi_throw_error(env, nullptr, "the particle system's solver was destroyed");
+    return nullptr;
+  }
+  return ph;
+}
+
+// particlesCreate(h, {count, trail, emitter: 0 inlet | 1 rake, nozzles, lines, maxAge, rake: Float32Array(12)})
+napi_value ParticlesCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  wt_particle_opts o{};
+  double v = 0;
+  o.count = get_prop_num(env, argv[1], "count", &v) ? (int32_t)v : 0;
+  o.trail = get_prop_num(env, argv[1], "trail", &v) ? (int32_t)v : 16;
+  o.emitter = get_prop_num(env, argv[1], "emitter", &v) ? (int32_t)v : 0;
+  o.nozzles = get_prop_num(env, argv[1], "nozzles", &v) ? (int32_t)v : 16;
+  o.lines = get_prop_num(env, argv[1], "lines", &v) ? (int32_t)v : 1;
+  o.max_age = get_prop_num(env, argv[1], "maxAge", &v) ? (float)v : 400.0f;
+  bool has = false;
+  napi_has_named_property(env, argv[1], "rake", &has);
+  if (has) {
+    napi_value rv;
+    napi_get_named_property(env, argv[1], "rake", &rv);
+    if (!get_rake(env, rv, &o.rake)) {
+      napi_throw_type_error(env, nullptr, "particlesCreate: rake must be a Float32Array(12)");
+      return nullptr;
+    }
+  }
+  wt_particles* p = nullptr;
+  int rc = wt_particles_create(h->s, &o, &p);
+  if (rc) return throw_wt(env, rc);
+  auto* ph = new PHandle{p, h, nullptr, o.count, o.trail};
+  napi_create_reference(env, argv[0], 1, &ph->owner_ref);
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, ph, finalize_particles, nullptr, &ext));
+  return ext;
+}
+
+napi_value ParticlesDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  void* p = nullptr;
+  napi_get_value_external(env, argv[0], &p);
+  auto* ph = static_cast<PHandle*>(p);
+  if (ph && ph->p) {
+    wt_particles_destroy(ph->p);
+    ph->p = nullptr;
+    if (ph->owner_ref) {
+      napi_delete_reference(env, ph->owner_ref);
+      ph->owner_ref = nullptr;
+    }
+  }
+  return nullptr;
+}
+
+// particlesUpdate(ph, steps, seed, fill, advance)
+napi_value ParticlesUpdate(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!args(env, info, 5, argv)) return nullptr;
+  PHandle* ph = get_particles(env, argv[0]);
+  if (!ph) return nullptr;
+  double steps = 0;
+  uint32_t seed = 0;
+  bool fill = false, advance = true;
+  napi_get_value_double(env, argv[1], &steps);
+  napi_get_value_uint32(env, argv[2], &seed);
+  napi_get_value_bool(env, argv[3], &fill);
+  napi_get_value_bool(env, argv[4], &advance);
+  int rc = wt_particles_update(ph->p, (float)steps, seed, fill ? 1 : 0, advance ? 1 : 0);
+  if (rc) return throw_wt(env, rc);
+  return nullptr;
+}
+
+// particlesRead(ph) -> {state: Float32Array(count x 4), trail: Float32Array(trail x count x 4), head}
+napi_value ParticlesRead(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  PHandle* ph = get_particles(env, argv[0]);
+  if (!ph) return nullptr;
+  void *ds, *dh;
+  napi_value st = make_typed(env, napi_float32_array, (size_t)ph->count * 4, 4, &ds);
+  napi_value tr = make_typed(env, napi_float32_array, (size_t)ph->trail * ph->count * 4, 4, &dh);
+  if (!st || !tr) {
+    napi_throw_error(env, nullptr, "out of memory");
+    return nullptr;
+  }
+  int32_t head = 0;
+  int rc = wt_particles_read(ph->p, (float*)ds, (float*)dh, &head);
+  if (rc) return throw_wt(env, rc);
+  napi_value out, hv;
+  napi_create_object(env, &out);
+  napi_create_int32(env, head, &hv);
+  napi_set_named_property(env, out, "state", st);
+  napi_set_named_property(env, out, "trail", tr);
+  napi_set_named_property(env, out, "head", hv);
+  return out;
+}
+
 napi_value BytesFor(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!args(env, info, 1, argv)) return nullptr;
@@ -547,7 +745,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"stepAndSample", StepAndSample}, {"stepAndSampleSync", StepAndSampleSync},
       {"readPopulations", ReadPopulations}, {"uploadPopulations", UploadPopulations},
       {"voxelizeSdf", VoxelizeSdf}, {"voxelizeMesh", VoxelizeMesh}, {"bytesFor", BytesFor},
-      {"derive", Derive}, {"exportFields", ExportFields},
+      {"derive", Derive}, {"exportFields", ExportFields}, {"probe", Probe}, {"streamlines", Streamlines},
+      {"particlesCreate", ParticlesCreate}, {"particlesDestroy", ParticlesDestroy},
+      {"particlesUpdate", ParticlesUpdate}, {"particlesRead", ParticlesRead},
   };
   for (auto& f : fns) {
     napi_value fn;
