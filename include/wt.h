@@ -286,7 +286,7 @@ int wt_copy_probe(int64_t bytes, int32_t iters, double* gbs);
 
 /* Launch tuning knobs (no reference counterpart; set by the autotuner).
  * WT_TUNE_ITEM_CHUNK: the bulk items (64 or 128-cell chunks of x-rows) are cut into runs of this
- * many consecutive items, dealt round-robin to the 8 XCDs; 4 is the plain order, 0 (default) gives
+ * many consecutive items, dealt round-robin to the 8 XCDs; 4 (default) is the plain order, 0 gives
  * every XCD one contiguous eighth of the grid.
  * WT_TUNE_CELLS_PER_THREAD: 2 (default; pairs along x, needs an even nx) or 1. FP16 / BF16
  * storage always uses 2. Changing it rebuilds the wall-cell list. */

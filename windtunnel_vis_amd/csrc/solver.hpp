@@ -59,7 +59,7 @@ struct wt_solver {
   int prof = 0;                 // WT_PROFILE_* mode
   std::vector<hipEvent_t> ev;   // event pairs: per launch (WT_PROFILE_LAUNCH) or per step call (WT_PROFILE_SPAN)
   std::vector<int64_t> ev_launches;  // launches covered by each pair
-  int item_chunk = 0;           // WT_TUNE_ITEM_CHUNK (0: one contiguous eighth per XCD)
+  int item_chunk = 4;           // WT_TUNE_ITEM_CHUNK (4: plain order, the fastest untuned choice; 0: eighths)
   int cpt = 2;                  // WT_TUNE_CELLS_PER_THREAD (2 needs an even nx)
   size_t ev_used = 0;
   double acc_bulk = 0, acc_bnd = 0;
