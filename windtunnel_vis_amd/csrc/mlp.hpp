@@ -121,6 +121,14 @@ __device__ inline uint32_t pack_bf16(float a, float b) {
 // max per operand besides).
 __device__ inline float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
+// max(x, 0) on two packed BF16 values with one packed 16-bit integer max (v_pk_max_i16). Rounding
+// then clamping gives the same bits as clamping then rounding (a negative value rounds to a
+// negative BF16 or to -0, both sent to +0).
+typedef short short2_hw __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t relu_bf16x2(uint32_t u) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(short2_hw, u), (short2_hw){0, 0}));
+}
+
 // Device-side network in the layout the MFMA path reads (prepared once from the FP32 master).
 struct alignas(16) MlpDev {
   bf16x4_t w1a[64];     // A operand of layer 1 per lane
@@ -160,8 +168,8 @@ __device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* 
     bf16x4_t hb[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      const uint32_t lo = pack_bf16(relu(acc[4 * c]), relu(acc[4 * c + 1]));
-      const uint32_t hi = pack_bf16(relu(acc[4 * c + 2]), relu(acc[4 * c + 3]));
+      const uint32_t lo = relu_bf16x2(pack_bf16(acc[4 * c], acc[4 * c + 1]));
+      const uint32_t hi = relu_bf16x2(pack_bf16(acc[4 * c + 2], acc[4 * c + 3]));
       hb[c] = __builtin_bit_cast(bf16x4_t, (uint64_t)lo | ((uint64_t)hi << 32));
     }
 #pragma unroll
