@@ -362,7 +362,9 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
         const int cx = L::c(i, 0), cy = L::c(i, 1);
         const bool grnd = cy > 0 && yLo && P.ground != 0;  // bounce-back: no x shift
         const bool edge = cx != 0 && !grnd && (cx > 0 ? first : last);
-        const uint32_t o = edge ? lane_off + (uint32_t)(cx * (int)sizeof(T)) : lane_off;
+        // signed: with nx = 2 the last pair starts at x0 = 0 and its aligned word lies one
+        // element before the shifted source (lane_off + c_x * sizeof(T) < 0)
+        const int32_t o = (int32_t)lane_off + (edge ? cx * (int32_t)sizeof(T) : 0);
         const Word v = S::ld_word((const T*)((const char*)src[i] + o));
         in.raw[i] = !edge ? v : (cx > 0 ? (Word)(v << HALF) : (Word)(v >> HALF));
       }
