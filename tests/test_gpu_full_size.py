@@ -32,15 +32,24 @@ def _roll_z(a, k):
     return np.roll(a.reshape(lead + (nz, ny, nx)), k, axis=-3).reshape(lead + (-1,))
 
 
-@pytest.mark.parametrize("prec", ["f16", "f32"])
-def test_shift_equivariance_periodic_z(prec):
+@pytest.mark.parametrize("prec,learned", [("f16", False), ("bf16", False), ("f32", False), ("f16", True)])
+def test_shift_equivariance_periodic_z(prec, learned):
+    """learned: the MLP closure's step kernel (features, wall distances, MFMA inference per
+    128-cell group) with a seeded random network, attached to both solvers."""
     from windtunnel_vis_amd import _native as N
+    from windtunnel_vis_amd import closure
     from windtunnel_vis_amd.geometry import prim
     p = dict(u=0.08, nu=6e-4, cs=0.1, ground="moving", sides="periodic", precision=prec)
     a = _solver(p)
     b = _solver(p)
     try:
         assert a.precision == prec and b.precision == prec
+        if learned:
+            net = closure.Network.initial(seed=5, nu_scale=1e-3)
+            net.mu = np.full(8, 1.0, np.float32)
+            net.sigma = np.full(8, 2.0, np.float32)
+            closure.attach(a, net)
+            closure.attach(b, net)
         # a sphere and a rounded box off the z centre, the box on the belt (ground contact)
         a.voxelizeSdf([prim(N.PRIM_SPHERE, (150.0, 120.0, 90.0), radius=30.0),
                        prim(N.PRIM_ROUNDED_BOX, (260.0, 20.0, 170.0), half=(40.0, 22.0, 30.0), radius=6.0)])
@@ -52,7 +61,7 @@ def test_shift_equivariance_periodic_z(prec):
             s.uploadFlags(fl)
             s.uploadSdf(d)
             s.reset()
-        print(f"[full-size equivariance] {prec} {DIMS} solid={solid} shift={SHIFT}", flush=True)
+        print(f"[full-size equivariance] {prec} learned={learned} {DIMS} solid={solid} shift={SHIFT}", flush=True)
         sa = a.stepAndSample(30)
         sb = b.stepAndSample(30)
         assert not sa.unstable and not sb.unstable
