@@ -1,15 +1,19 @@
 """Benchmark: D3Q19 Smagorinsky LES MLUPS and HBM-roofline fraction on one MI355X.
 
-Workload (default): the plain D3Q19 LES at 256x128x128 with FP16 deviation storage, Bouzidi walls,
-moving belt at u = 0.08 with rolling wheels, Cs = 0.1, around the procedural sedan (rounded box
-85x22x32 cells, corner radius 6, four wheels r = 6, width 5, ride height 3). One step = one fused
+Headline workload (default): the plain D3Q19 LES at 512x256x256 with FP16 deviation storage,
+Bouzidi walls, moving belt at u = 0.08 with rolling wheels, Cs = 0.1, around the procedural sedan
+(rounded box 85x22x32 cells, corner radius 6, four wheels r = 6, width 5, ride height 3, nose at
+0.34 nx). At this size the 2.5 GB ping-pong is ten times the 256 MB memory-side
+cache, so the HBM roofline fraction is the memory system's, not the cache's. One step = one fused
 pull-stream + collide pass over the whole grid, one launch of the step kernel: its first blocks
 update the cells next to the body from precomputed links, the rest stream the bulk.
 
 Prints ONE JSON line. `value` is MLUPS over the timed steps with the populations resident in HBM.
 `roofline` prices the step kernel at 2*Q*s algorithmic bytes per fluid cell, divided by its average
-launch time measured with HIP events on the solver's stream, against the 8 TB/s HBM peak.
-`cpu_baseline` times the CPU restatement on a bounded sample.
+launch time measured with HIP events on the solver's stream, against the 8 TB/s HBM peak; `traffic`
+is the HBM bytes per launch from the committed rocprofv3 counter summary of the same workload.
+`cpu_baseline` times the CPU restatement on a bounded sample of the same workload. The 256x128x128
+line (the sedan LES configuration with its 5000-step C_D / C_L) and the other workloads are extra keys.
 """
 import argparse
 import json
@@ -319,44 +323,68 @@ def closure_bench():
                              "converged-window C_D"}
 
 
-def cpu_baseline(dims, sample_steps, threads):
-    """The CPU restatement (FP32 arithmetic path, OpenMP) on the same domain for a few steps."""
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU this process may run on, capped by OMP_NUM_THREADS
+    when that is set (the GPU box gives one GPU's job a 16-core share and sets it to 16)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(avail, int(cap))) if cap and cap.isdigit() else avail
+
+
+def cpu_baseline(solver, sample_steps, threads):
+    """The CPU restatement (FP32 arithmetic path with FP16 store rounding, OpenMP) on the same domain:
+    the GPU solver's own flags (wheel tags included), wall distances (Bouzidi links) and wheels,
+    timed over a few steps."""
     from oracle.oracle import OracleSolver
-    from windtunnel_vis_amd import geometry as G
-    p = workload_params(precision="f32")
+    dims = (solver.dims["nx"], solver.dims["ny"], solver.dims["nz"])
+    p = dict(solver.params_)
+    p["precision"] = "f32"
     o = OracleSolver(dims, p, storage="f16", fp32=True, threads=threads)
-    body = G.sedan(dims)
-    nx, ny, nz = dims
-    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
-    d = body.sdf(np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1))
-    flags = (d < 0).astype(np.uint32)
-    o.setWheels(body.wheels)
-    o.uploadFlags(flags)
+    o.setWheels(solver.wheels_)
+    o.uploadFlags(solver.readFlags())
+    o.uploadSdf(solver.readSdf())
     o.step(1)  # init + first touch
     t0 = time.perf_counter()
     o.step(sample_steps)
     dt = time.perf_counter() - t0
-    n = nx * ny * nz
+    o.destroy()
+    n = dims[0] * dims[1] * dims[2]
     return dict(value=n * sample_steps / dt / 1e6, seconds=dt)
+
+
+def roofline_block(r, copy_gbs, dims, traffic_name):
+    return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(traffic_name),
+            "kernel": "k_step<D3Q19,SF16,2,regularized>", "bytes_per_launch": r["k_bytes"],
+            "avg_launch_ms": round(r["k_avg_ms"], 5), "fluid_cells_per_launch": r["fluid"],
+            "timing": "HIP event pair on the solver's stream around the timed launches: busy span / launches",
+            "avg_launch_ms_events_per_launch": round(r["k_avg_ms_launch_events"], 5),
+            "measured_copy_gbs": round(copy_gbs, 1)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--dims", default="256,128,128")
+    ap.add_argument("--dims", default="512,256,256", help="headline grid (the sedan LES workload)")
     ap.add_argument("--precision", default="f16")
     ap.add_argument("--no-physics", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU-baseline sample steps (0 = auto)")
-    ap.add_argument("--large", action="store_true", help="also time 512x256x256 (the default; kept for old command lines)")
-    ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 and slab lines")
+    ap.add_argument("--no-small", action="store_true", help="skip the 256x128x128 line (its physics and the learned step)")
+    ap.add_argument("--no-large", action="store_true", help="skip the x-slab shard line")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched 24-domain sweep")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure study")
     ap.add_argument("--no-stress", action="store_true", help="skip the 1024x512x512 precision stress")
     ap.add_argument("--profile-run", action="store_true",
-                    help="short run for counter collection: copy probe + the main workload only")
+                    help="short run for counter collection: copy probe + the headline workload only")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="bytes of the copy probe in MiB (0: 256 MiB with --profile-run, else 2048)")
     ap.add_argument("--no-learned-step", action="store_true", help="skip timing the learned-closure step")
     ap.add_argument("--learned-step", action="store_true", help="time the learned-closure step also with --profile-run")
     ap.add_argument("--chunk", type=int, default=None,
@@ -366,11 +394,12 @@ def main():
         print("bench.py measures one GPU only: there is no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     dims = tuple(int(v) for v in args.dims.split(","))
-    args.large = not args.no_large and dims != (512, 256, 256)
+    small = (256, 128, 128)
     if args.profile_run:
-        args.no_physics = args.no_cpu = args.no_sweep = args.no_closure = args.no_stress = True
-        args.large = False
-    copy_gbs = copy_probe(256 << 20, 3) if args.profile_run else copy_probe()
+        args.no_physics = args.no_cpu = args.no_sweep = args.no_closure = args.no_stress = args.no_large = True
+        args.no_small = not args.learned_step
+    copy_mib = args.copy_mib or (256 if args.profile_run else 2048)
+    copy_gbs = copy_probe(copy_mib << 20, 3 if args.profile_run else 5)
     r = run_gpu(dims, args.steps, args.warmup, args.precision, chunk=args.chunk)
     out = {
         "metric": METRIC, "value": round(r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
@@ -380,15 +409,7 @@ def main():
                                "Bouzidi walls, moving belt + rolling wheels, u=0.08, Cs=0.1" % (dims + (args.precision.upper(),)),
                    "grid": list(dims), "storage": args.precision, "collision": "regularized", "closure": "smagorinsky"},
     }
-    frac = r["achieved_gbs"] / HBM_PEAK_GBS
-    out["roofline"] = {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(frac, 4),
-                       "traffic": traffic_from_profiles("pmc_step_d3q19_f16_%dx%dx%d.txt" % dims),
-                       "kernel": "k_step<D3Q19,SF16,2,regularized>", "bytes_per_launch": r["k_bytes"],
-                       "avg_launch_ms": round(r["k_avg_ms"], 5), "fluid_cells_per_launch": r["fluid"],
-                       "timing": "HIP event pair on the solver's stream around the timed launches: busy span / launches",
-                       "avg_launch_ms_events_per_launch": round(r["k_avg_ms_launch_events"], 5),
-                       "measured_copy_gbs": round(copy_gbs, 1)}
+    out["roofline"] = roofline_block(r, copy_gbs, dims, "pmc_step_d3q19_%s_%dx%dx%d.txt" % ((args.precision,) + dims))
     out["step_roofline_frac"] = round(r["mlups"] * 1e6 * 2 * r["q"] * r["sbytes"] / (HBM_PEAK_GBS * 1e9), 4)
     out["work_split"] = {"bulk_fluid_cells": r["bulk_cells"], "wall_cells": r["wall_cells"]}
     if r["tune"]:
@@ -397,19 +418,34 @@ def main():
                          "trials_ms_per_step": {"%d/%d" % (e["cellsPerThread"], e["chunk"]): round(e["msPerStep"], 5)
                                                 for e in t["trials"]}}
     out["unstable"] = bool(r["unstable"])
-    if not args.no_physics:
-        ph = physics(r["solver"], r["body"], r["info"], 0.08)
-        out["physics"] = {"cd": round(ph["cd"], 4), "cl": round(ph["cl"], 4), "frontal_cells": ph["frontal"],
-                          "unstable": ph["unstable"], "steps": 5000}
+    if not args.no_cpu:
+        threads = cpu_threads()
+        steps = args.cpu_steps or max(2, int(round(12 * 33554432 / (dims[0] * dims[1] * dims[2]))))
+        cb = cpu_baseline(r["solver"], steps, threads)
+        out["cpu_baseline"] = {"value": round(cb["value"], 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+                               "host_cpus": os.cpu_count(),
+                               "sample": "%d steps of the same %dx%dx%d sedan domain (the GPU run's flags, wall distances and "
+                                         "wheels: Bouzidi links, rolling wheels, belt), FP32 CPU restatement with FP16 store "
+                                         "rounding, OpenMP on %d threads, %.1f s" % ((steps,) + dims + (threads, cb["seconds"]))}
     r["solver"].destroy()
-    if args.large:
-        big = (512, 256, 256)
-        rb = run_gpu(big, max(args.steps // 4, 200), args.warmup, args.precision)
-        out["d3q19_f16_512x256x256"] = {"mlups": round(rb["mlups"], 1), "chunk": rb["tune"]["chunk"],
-                                          "step_roofline_frac": round(rb["mlups"] * 1e6 * 2 * rb["q"] * rb["sbytes"] / (HBM_PEAK_GBS * 1e9), 4),
-                                          "kernel_frac": round(rb["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                                          "avg_launch_ms": round(rb["k_avg_ms"], 5)}
-        rb["solver"].destroy()
+    if not args.no_small and dims != small:
+        rs = run_gpu(small, args.steps, args.warmup, args.precision, chunk=args.chunk)
+        line = {"mlups": round(rs["mlups"], 1), "ms_per_step": round(rs["wall"] * 1e3 / args.steps, 5),
+                "chunk": rs["tune"]["chunk"] if rs["tune"] else args.chunk,
+                "kernel_frac": round(rs["achieved_gbs"] / HBM_PEAK_GBS, 4), "avg_launch_ms": round(rs["k_avg_ms"], 5),
+                "step_roofline_frac": round(rs["mlups"] * 1e6 * 2 * rs["q"] * rs["sbytes"] / (HBM_PEAK_GBS * 1e9), 4),
+                "traffic": traffic_from_profiles("pmc_step_d3q19_%s_%dx%dx%d.txt" % ((args.precision,) + small)),
+                "note": "the 318.8 MB ping-pong partly fits the 256 MB memory-side cache: not a roofline figure"}
+        if not args.no_physics:
+            ph = physics(rs["solver"], rs["body"], rs["info"], 0.08)
+            line["physics"] = {"cd": round(ph["cd"], 4), "cl": round(ph["cl"], 4), "frontal_cells": ph["frontal"],
+                               "unstable": ph["unstable"], "steps": 5000}
+        out["d3q19_%s_256x128x128" % args.precision] = line
+        plain_ms = rs["k_avg_ms"]
+        rs["solver"].destroy()
+        if args.learned_step or not (args.no_learned_step or args.profile_run):
+            out["learned_step"] = learned_step_bench(small, max(min(args.steps, 2000) // 4, 50), args.warmup, plain_ms,
+                                                     args.precision)
     if not args.no_physics:
         from windtunnel_vis_amd.validation import cylinder_512x128
         from windtunnel_vis_amd import SolverGPU
@@ -418,24 +454,14 @@ def main():
                         "cycles": w["cycles"], "steps": w["steps"]}
         out["cylinder_512x128"] = {"window_20k": rd(c1["window_20k"]), "developed": rd(c1["developed"]),
                                    "unstable": c1["unstable"], "seconds": round(c1["seconds"], 2)}
-    if not (args.no_large or args.profile_run):
+    if not args.no_large:
         out["shard_128x256x256"] = shard_bench(max(min(args.steps, 2000) // 4, 100), args.warmup, args.precision)
-    if args.learned_step or not (args.no_learned_step or args.profile_run):
-        out["learned_step"] = learned_step_bench(dims, max(min(args.steps, 2000) // 4, 50), args.warmup, r["k_avg_ms"],
-                                                 args.precision)
     if not args.no_sweep:
         out["batched_sweep"] = sweep_bench()
     if not args.no_closure:
         out["learned_closure"] = closure_bench()
     if not args.no_stress:
         out["precision_stress_1024x512x512"] = precision_stress()
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        steps = args.cpu_steps or 100
-        cb = cpu_baseline(dims, steps, threads)
-        out["cpu_baseline"] = {"value": round(cb["value"], 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-                               "sample": "%d steps of the same %dx%dx%d sedan domain (FP32 CPU restatement, FP16 store "
-                                         "rounding, OpenMP), %.1f s" % ((steps,) + dims + (cb["seconds"],))}
     print(json.dumps(out))
 
 

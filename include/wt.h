@@ -317,9 +317,11 @@ int wt_autotune(wt_solver* s, int32_t steps, wt_tune_result* out);
  * parameters, layout and buffers. Steps run as replays of a hipGraph-captured chunk (default 100
  * steps) plus direct launches for the remainder. The members must share lattice, storage
  * precision, cells per thread and collision, must have been stepped equally often, and stay
- * owned by the caller (destroy the batch first). Geometry, parameter and tuning changes on a
- * member are picked up at the next wt_batch_run. Per-member statistics are read with
- * wt_batch_stats (one wt_stats per member, in member order). */
+ * owned by the caller (destroy the batch first). Geometry, parameter and item-order changes on a
+ * member are picked up at the next wt_batch_run; a change that breaks the shared kernel variant
+ * (cells per thread, collision operator, a loaded network with the learned closure, which the
+ * batched step does not run) makes wt_batch_run fail with WT_E_STATE and step nothing. Per-member
+ * statistics are read with wt_batch_stats (one wt_stats per member, in member order). */
 typedef struct wt_batch wt_batch;
 int wt_batch_create(wt_solver* const* members, int32_t count, wt_batch** out);
 int wt_batch_destroy(wt_batch* b);

@@ -45,6 +45,21 @@ struct or_params {
   int32_t threads;     // OpenMP threads (0 = default)
 };
 
+// The learned closure (no reference counterpart: it replaces the Smagorinsky tau of
+// wgslLbm.ts:214), as the product defines it (include/wt.h, wt_mlp): an 8-32-32-1 network on six
+// strain invariants, the wall distance and the local Reynolds number, BF16 inputs, weights and
+// first hidden layer, softplus output capped at 10 x nu_scale, guarded by smag_floor x the
+// Smagorinsky eddy viscosity. It runs on bulk cells: fluid cells whose group of `pair` cells along
+// x (the product's cells per lane) holds no cell with a plain pull from a solid and no solid cell;
+// the other cells keep Smagorinsky.
+struct or_mlp {
+  double params[1377];  // W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3
+  double mu[8], sigma[8];
+  double nu_scale, smag_floor;
+  int32_t pair;          // 1 or 2 (2 needs an even nx)
+  int32_t pad;
+};
+
 }  // extern "C"
 
 namespace {
@@ -120,6 +135,51 @@ double round_bf16(double x) {
   return std::copysign(r, x);
 }
 
+// Inputs of the closure network: six rotation invariants of the strain rate S (tr S, the second and
+// third principal invariants, tr S^2, tr S^3, |S| = sqrt(2 tr S^2); each as sign(x) log(1 + |x| /
+// 1e-3^degree)), the wall distance and log(1 + |u| max(d, 1/2) / nu0). S: sxx, syy, szz, sxy, sxz, syz.
+void mlp_features(const double S[6], double wd, double umag, double nu0, double x[8]) {
+  const double sxx = S[0], syy = S[1], szz = S[2], sxy = S[3], sxz = S[4], syz = S[5];
+  const double i1 = sxx + syy + szz;
+  const double tr2 = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+  const double i2 = 0.5 * (i1 * i1 - tr2);
+  const double i3 = sxx * (syy * szz - syz * syz) - sxy * (sxy * szz - syz * sxz) + sxz * (sxy * syz - syy * sxz);
+  const double tr3 = i1 * i1 * i1 - 3.0 * i1 * i2 + 3.0 * i3;
+  const double m = std::sqrt(2.0 * tr2);
+  auto slog = [](double v, double ref) { return std::copysign(std::log1p(std::fabs(v) / ref), v); };
+  const double r = 1e-3;
+  x[0] = slog(i1, r);
+  x[1] = slog(i2, r * r);
+  x[2] = slog(i3, r * r * r);
+  x[3] = slog(tr2, r * r);
+  x[4] = slog(tr3, r * r * r);
+  x[5] = slog(m, r);
+  x[6] = wd;
+  x[7] = std::log1p(umag * std::max(wd, 0.5) / nu0);
+}
+
+// The network on normalised inputs with the MFMA path's roundings: BF16 inputs, weights and
+// (after the bias, before the ReLU) first hidden layer; accumulation, second layer and output in
+// FP64 here. Returns nu_t = nu_scale * min(softplus(z), 10).
+double mlp_forward(const or_mlp& net, const double xn[8]) {
+  const double* p = net.params;
+  double h1[32], h2[32];
+  for (int o = 0; o < 32; o++) {
+    double a = p[256 + o];
+    for (int i = 0; i < 8; i++) a += round_bf16(p[o * 8 + i]) * round_bf16(xn[i]);
+    h1[o] = std::max(round_bf16(a), 0.0);
+  }
+  for (int o = 0; o < 32; o++) {
+    double a = p[1312 + o];
+    for (int i = 0; i < 32; i++) a += round_bf16(p[288 + o * 32 + i]) * h1[i];
+    h2[o] = std::max(a, 0.0);
+  }
+  double z = p[1376];
+  for (int i = 0; i < 32; i++) z += p[1344 + i] * h2[i];
+  const double sp = z > 20.0 ? z : std::log1p(std::exp(z));
+  return net.nu_scale * std::min(sp, 10.0);
+}
+
 template <class T>
 struct Solver {
   or_params prm;
@@ -138,6 +198,38 @@ struct Solver {
   bool pending_init = true;
   int64_t step = 0;
   std::vector<T> meanA, meanB, vel;
+  // learned closure: network, wall distance (cells x 32) and the cells it runs on
+  bool mlp_on = false;
+  or_mlp net;
+  std::vector<uint8_t> wdist, use_net;
+
+  // Cells the closure network runs on (see or_mlp): fluid, and no cell of the pair group is solid
+  // or has a plain pull (wgslLbm.ts:172) from a solid cell.
+  void classify_net() {
+    std::vector<uint8_t> bad(n, 0);
+    const int Q = L.q, dim = L.dim;
+    for (int64_t idx = 0; idx < n; idx++) {
+      if (flags[idx] != 0u) { bad[idx] = 1; continue; }
+      const int ix = (int)(idx % nx), iy = (int)((idx / nx) % ny), iz = (int)(idx / ((int64_t)nx * ny));
+      for (int i = 1; i < Q; i++) {
+        const int cx = L.c[i][0], cy = L.c[i][1], cz = L.c[i][2];
+        int sx = ix - cx;
+        const int sy = iy - cy, sz = dim == 3 ? iz - cz : 0;
+        if (cx < 0) sx = std::min(sx, nx - 1);
+        if (cx > 0 && sx < 0) continue;                                 // inlet
+        if (sy < 0 || sy >= ny || sz < 0 || sz >= nz) continue;         // ground, mirrors, periodic
+        if (flags[cell(sx, sy, sz)] != 0u) { bad[idx] = 1; break; }
+      }
+    }
+    const int pair = (net.pair == 2 && nx % 2 == 0) ? 2 : 1;
+    use_net.assign(n, 0);
+    for (int64_t idx = 0; idx < n; idx++) {
+      const int64_t g0 = idx - idx % pair;
+      bool ok = true;
+      for (int64_t j = g0; j < g0 + pair; j++) ok = ok && !bad[j];
+      use_net[idx] = ok ? 1 : 0;
+    }
+  }
 
   void derive() {
     tau0 = T(3.0 * prm.nu + 0.5);
@@ -335,6 +427,22 @@ struct Solver {
     T t0 = tau0 + T(3) * spongeNu * sp * sp;
     if (gwall) t0 = std::max(t0, tauWall);
     T tau = T(0.5) * (t0 + std::sqrt(t0 * t0 + T(18.0 * std::sqrt(2.0)) * csmag2 * piN * irho));
+    if (mlp_on && dim == 3 && use_net[idx]) {
+      // learned closure: strain rate from the non-equilibrium stress with the Smagorinsky tau,
+      // S = -3 Pi^neq / (2 rho tau); nu_t = max(network, floor x nu_smag); a non-finite (or huge)
+      // output keeps Smagorinsky
+      const double sc = -1.5 / ((double)rho * (double)tau);
+      const double S6[6] = {sc * (double)Pi[0][0], sc * (double)Pi[1][1], sc * (double)Pi[2][2],
+                            sc * (double)Pi[0][1], sc * (double)Pi[0][2], sc * (double)Pi[1][2]};
+      double x[8], xn[8];
+      mlp_features(S6, wdist[idx] / 32.0, std::sqrt((double)usq), ((double)tau0 - 0.5) / 3.0, x);
+      for (int k = 0; k < 8; k++) xn[k] = (x[k] - net.mu[k]) / (net.sigma[k] > 0 ? net.sigma[k] : 1.0);
+      const double nu = mlp_forward(net, xn);
+      if (nu == nu && nu < 1e3) {
+        const double nus = ((double)tau - (double)t0) / 3.0;
+        tau = t0 + T(3.0 * std::max(nu, net.smag_floor * nus));
+      }
+    }
     T k1 = (T(1) - T(1) / tau) * T(4.5);
     bool useBGK = wall || collision == 1;
     T si = std::min(std::max(T(1) - fx / std::max(spongeIn, T(1)), T(0)), T(1));
@@ -360,6 +468,7 @@ struct Solver {
 
   void do_step(int steps) {
     if (pending_init) do_init();
+    if (mlp_on) classify_net();
     const int Q = L.q;
     for (int s = 0; s < steps; s++) {
       const T* fin = cur();
@@ -607,6 +716,33 @@ void or_read_macro(void* hp, double* vel, double* meanA, double* meanB) {
     for (int64_t i = 0; i < 4 * s->n; i++) { vel[i] = (double)s->vel[i]; meanA[i] = (double)s->meanA[i]; }
     for (int64_t i = 0; i < s->n; i++) meanB[i] = (double)s->meanB[i];
   });
+}
+
+// Attach the learned closure (net == NULL detaches it). wdist: n bytes, wall distance x 32.
+void or_set_mlp(void* hp, const or_mlp* net, const uint8_t* wdist) {
+  OR_DISPATCH({
+    if (!net) {
+      s->mlp_on = false;
+    } else {
+      s->net = *net;
+      s->wdist.assign(wdist, wdist + s->n);
+      s->mlp_on = true;
+    }
+  });
+}
+
+// The closure network alone (n x 8 raw inputs -> n eddy viscosities), for checking the restatement.
+void or_mlp_infer(const or_mlp* net, const double* raw, int64_t n, double* nu) {
+  for (int64_t c = 0; c < n; c++) {
+    double xn[8];
+    for (int k = 0; k < 8; k++) xn[k] = (raw[8 * c + k] - net->mu[k]) / (net->sigma[k] > 0 ? net->sigma[k] : 1.0);
+    nu[c] = mlp_forward(*net, xn);
+  }
+}
+
+// mlp_features for n cells (S: n x 6), for checking the restatement.
+void or_mlp_features(const double* S, const double* wd, const double* umag, int64_t n, double nu0, double* x) {
+  for (int64_t c = 0; c < n; c++) mlp_features(S + 6 * c, wd[c], umag[c], nu0, x + 8 * c);
 }
 
 double or_round_half(double x) { return round_half(x); }

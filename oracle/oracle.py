@@ -30,6 +30,24 @@ class OrParams(C.Structure):
     ]
 
 
+class OrMlp(C.Structure):
+    _fields_ = [("params", C.c_double * 1377), ("mu", C.c_double * 8), ("sigma", C.c_double * 8),
+                ("nu_scale", C.c_double), ("smag_floor", C.c_double), ("pair", C.c_int32), ("pad", C.c_int32)]
+
+
+def mlp_struct(net, pair=2):
+    """OrMlp from a closure network (windtunnel_vis_amd.closure.Network or anything with params,
+    mu, sigma, nu_scale, smag_floor)."""
+    m = OrMlp()
+    m.params[:] = [float(v) for v in np.asarray(net.params, np.float64)]
+    m.mu[:] = [float(v) for v in np.asarray(net.mu, np.float64)]
+    m.sigma[:] = [float(v) for v in np.asarray(net.sigma, np.float64)]
+    m.nu_scale = float(net.nu_scale)
+    m.smag_floor = float(net.smag_floor)
+    m.pair = int(pair)
+    return m
+
+
 _lib = None
 
 
@@ -63,6 +81,9 @@ def load():
         "or_read_macro": (None, [P, P, P, P]),
         "or_round_half": (C.c_double, [C.c_double]),
         "or_round_bf16": (C.c_double, [C.c_double]),
+        "or_set_mlp": (None, [P, P, P]),
+        "or_mlp_infer": (None, [P, P, C.c_int64, P]),
+        "or_mlp_features": (None, [P, P, P, C.c_int64, C.c_double, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -199,6 +220,39 @@ class OracleSolver:
         mb = np.empty(self.n)
         self._lib.or_read_macro(self._h, _ptr(vel), _ptr(ma), _ptr(mb))
         return vel, ma, mb
+
+    def setMlp(self, net, wdist, pair=2):
+        """Attach the learned closure: `net` a closure network, `wdist` the wall distance in cells
+        (n values, stored as round(32 d) bytes like the product), `pair` the product's cells per
+        lane (the closure runs on groups of `pair` cells along x with no wall or solid cell).
+        net None detaches it."""
+        if net is None:
+            self._lib.or_set_mlp(self._h, None, None)
+            return
+        self._mlp = mlp_struct(net, pair)
+        self._wd = np.ascontiguousarray(np.rint(np.asarray(wdist, np.float64).reshape(-1) * 32), dtype=np.uint8)
+        self._lib.or_set_mlp(self._h, C.byref(self._mlp), _ptr(self._wd))
+
+
+def mlp_infer(net, raw):
+    """The closure network alone on raw inputs (n, 8): eddy viscosities (n,) in FP64 with the MFMA
+    path's BF16 roundings."""
+    r = np.ascontiguousarray(raw, dtype=np.float64).reshape(-1, 8)
+    out = np.empty(len(r))
+    m = mlp_struct(net)
+    load().or_mlp_infer(C.byref(m), _ptr(r), len(r), _ptr(out))
+    return out
+
+
+def mlp_features(S6, wd, umag, nu0):
+    """The eight raw closure inputs of n cells from their strain rates (n, 6), wall distances and speeds."""
+    S = np.ascontiguousarray(S6, dtype=np.float64).reshape(-1, 6)
+    n = len(S)
+    w = np.ascontiguousarray(np.broadcast_to(np.asarray(wd, np.float64), (n,)))
+    u = np.ascontiguousarray(np.broadcast_to(np.asarray(umag, np.float64), (n,)))
+    out = np.empty((n, 8))
+    load().or_mlp_features(_ptr(S), _ptr(w), _ptr(u), n, float(nu0), _ptr(out))
+    return out
 
 
 def round_half(x):

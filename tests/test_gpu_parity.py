@@ -137,10 +137,7 @@ def test_store_rounds_to_nearest_even(prec):
         ulp = 2.0 ** -17  # spacing of bfloat16 in [2^-10, 2^-9)
         base = 2.0 ** -10
     devs = np.array([base + 0.5 * ulp, base + 1.5 * ulp, base + 0.5 * ulp + ulp / 64, -(base + 0.5 * ulp)] * 8, dtype=np.float64)
-    expect = np.array([base, base + 2 * ulp, base + ulp, -base] * 8)
     f = (w[:, None].astype(np.float64) + devs[None, :]).astype(np.float32)
-    # the float32 sum must still carry the exact deviation for this test to be meaningful
-    assert np.all((f.astype(np.float64) - w[:, None]) == devs[None, :]) or True
     g.uploadPopulations(f)
     got = g.readPopulations().astype(np.float64) - w[:, None].astype(np.float64)
     exact = f.astype(np.float64) - w[:, None].astype(np.float32).astype(np.float64)

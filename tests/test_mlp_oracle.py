@@ -119,3 +119,71 @@ def test_bf16_forward_close_to_fp64():
     a = M.forward(p, x, 1e-3)[0]
     b = M.forward_bf16(p, x, 1e-3)
     assert np.max(np.abs(a - b) / (np.abs(a) + 1e-4)) < 0.05
+
+
+def _linear_target(G):
+    """Closed form of the training target for a fine field u = G x (G[i, k] = du_i / dx_k): the 2x2x2
+    block variance of each coordinate is 1/4, so tau = G G^T / 4; block means advance by 2 G per coarse
+    cell, so the filtered strain (coarse units) is S = G + G^T; nu_t = max(0, -tau_dev : S / (2 S : S))
+    (uniform over the 3x3x3 neighbourhood)."""
+    tau = G @ G.T / 4.0
+    S = G + G.T
+    tdev = tau - np.trace(tau) / 3.0 * np.eye(3)
+    return max(0.0, -np.sum(tdev * S) / (2.0 * np.sum(S * S))), S
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_coarse_samples_linear_field_nonzero_target(seed):
+    """A linear fine field u = u0 + G x with random G: the target has the closed form above, which is
+    non-zero for about half of the draws (a flipped sign of the numerator sends those to 0 and the
+    others to a positive value; a factor 2 in tau, S or the denominator scales it)."""
+    rng = np.random.default_rng(100 + seed)
+    G = rng.normal(size=(3, 3)) * 1e-3
+    dims = (10, 12, 8)
+    z, y, x = np.meshgrid(np.arange(8) + 0.5, np.arange(12) + 0.5, np.arange(10) + 0.5, indexing="ij")
+    X = np.stack([x, y, z], axis=-1).reshape(-1, 3)
+    u = np.array([0.05, 0.01, -0.02]) + X @ G.T
+    solid = np.zeros(len(u), bool)
+    wd = np.full(5 * 6 * 4, 4.0)
+    feats, tgt, idx = M.coarse_samples(u, solid, dims, wd, 6e-4)
+    assert len(idx) == 3 * 4 * 2
+    want, S = _linear_target(G)
+    assert np.allclose(tgt, want, rtol=1e-9, atol=1e-18)
+    S6 = np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]])
+    cx, cy = 5, 6
+    c = np.stack([idx % cx, idx // cx % cy, idx // (cx * cy)], axis=1) * 2.0 + 1.0  # block centres, fine units
+    ub = np.linalg.norm(np.array([0.05, 0.01, -0.02]) + c @ G.T, axis=1)
+    ref = M.raw_features(np.repeat(S6[None], len(idx), 0), np.full(len(idx), 4.0), ub, 6e-4)
+    assert np.allclose(feats, ref, rtol=1e-9, atol=1e-12)
+
+
+def test_coarse_samples_nonzero_target_example():
+    # u_x = a (x + y), a < 0: tau_xx = a^2 / 2, S_xx = 2a, S_xy = a, so nu_t = -a / 18 > 0; a > 0 gives 0
+    for a, want in ((-1e-3, 1e-3 / 18), (1e-3, 0.0)):
+        G = np.zeros((3, 3))
+        G[0, 0] = G[0, 1] = a
+        assert _linear_target(G)[0] == pytest.approx(want, rel=1e-12, abs=1e-20)
+        dims = (8, 8, 8)
+        z, y, x = np.meshgrid(np.arange(8) + 0.5, np.arange(8) + 0.5, np.arange(8) + 0.5, indexing="ij")
+        u = np.stack([0.05 + a * (x + y), 0 * x, 0 * x], axis=-1).reshape(-1, 3)
+        _, tgt, _ = M.coarse_samples(u, np.zeros(512, bool), dims, np.full(64, 4.0), 6e-4)
+        assert np.allclose(tgt, want, rtol=1e-9, atol=1e-20)
+
+
+def test_closure_restatements_agree():
+    """The C++ oracle's closure network and features equal the numpy restatement (both FP64, the same
+    BF16 roundings)."""
+    from oracle import oracle as O
+    from windtunnel_vis_amd import closure
+    rng = np.random.default_rng(12)
+    net = closure.Network.initial(seed=3, nu_scale=4e-3, smag_floor=0.0)
+    net.mu = rng.normal(size=8).astype(np.float32)
+    net.sigma = np.full(8, 3.0, np.float32)
+    raw = rng.normal(size=(3000, 8)) * 3
+    a = O.mlp_infer(net, raw)
+    b = M.forward_bf16(net.params.astype(np.float64), (raw - net.mu) / net.sigma, net.nu_scale)
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
+    S = rng.normal(size=(500, 6)) * 1e-3
+    wd = rng.uniform(0, 4, 500)
+    um = rng.uniform(0, 0.1, 500)
+    assert np.allclose(O.mlp_features(S, wd, um, 6e-4), M.raw_features(S, wd, um, 6e-4), rtol=1e-13, atol=1e-13)

@@ -200,7 +200,8 @@ def check(code):
 
 def ptr(a):
     """Raw data pointer of a C-contiguous numpy array."""
-    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
     return C.c_void_p(a.ctypes.data)
 
 

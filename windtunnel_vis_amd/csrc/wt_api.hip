@@ -223,12 +223,29 @@ void batch_drop_graph(wt_batch* b) {
   }
 }
 
+// The batched kernel variant is fixed by the batch's lattice, storage, cells per thread and
+// collision; a member whose state no longer matches it (retuned, switched collision, learned
+// closure attached: k_step_batch has no network path) cannot be stepped by it.
+int batch_member_check(const wt_solver* s, int q, int prec, int cpt, bool reg) {
+  if (s->q != q || s->prec != prec) return fail(WT_E_ARG, "batch members must share lattice and storage precision");
+  if (cells_per_thread(s->P) != cpt)
+    return fail(WT_E_STATE, "a batch member's cells per thread differs from the batch's (retuned after the batch was created)");
+  if ((s->P.collision != 1) != reg)
+    return fail(WT_E_STATE, "a batch member's collision operator differs from the batch's (changed after the batch was created)");
+  if (s->P.closure == WT_CLOSURE_MLP && s->mlp)
+    return fail(WT_E_STATE, "a batch member uses the learned closure, which the batched step does not run");
+  return WT_OK;
+}
+
 // Descriptor table and block -> domain map for the members' current state; uploads and drops the
 // captured graph when anything changed.
 int batch_prepare(wt_batch* b) {
   std::vector<DomainDesc> desc(b->m.size());
   uint32_t wall = 0;
   std::vector<uint32_t> nbb(b->m.size());
+  for (size_t k = 0; k < b->m.size(); k++) {
+    if (int r = batch_member_check(b->m[k], b->q, b->prec, b->cpt, b->reg)) return r;
+  }
   for (size_t k = 0; k < b->m.size(); k++) {
     wt_solver* s = b->m[k];
     LaunchCtx c = s->ctx();
@@ -297,10 +314,11 @@ int wt_create(const wt_dims* dims, const wt_params* params, wt_solver** out) {
   const int64_t n = (int64_t)dims->nx * dims->ny * dims->nz;
   if (n >= (1ll << 31)) return fail(WT_E_ARG, "grid too large: cell count must be below 2^31");
   if (int r = check_params(params)) return r;
-  int ndev = 0;
+  int ndev = 0, dev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(WT_E_NODEV, "no HIP device available");
+  HIP_TRY(hipGetDevice(&dev));
   auto* s = new wt_solver();
-  HIP_TRY(hipGetDevice(&s->device));
+  s->device = dev;
   s->dims = *dims;
   s->n = n;
   s->q = dims->nz > 1 ? 19 : 9;
@@ -330,15 +348,22 @@ int wt_create(const wt_dims* dims, const wt_params* params, wt_solver** out) {
     wt_destroy(s);
     return fail(WT_E_NOMEM, "pinned host allocation failed");
   }
+  std::memset(s->h_result, 0, 64);  // wt_sample_raw before the first sample reads zeros
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     wt_destroy(s);
     return fail(WT_E_HIP, "hipStreamCreate failed");
   }
-  HIP_TRY(hipMemsetAsync(s->flags, 0, 4 * n, s->stream));
-  HIP_TRY(hipMemsetAsync(s->sdf, 0, 4 * n, s->stream));
-  HIP_TRY(hipMemsetAsync(s->f[0], 0, fbytes + 64, s->stream));
-  HIP_TRY(hipMemsetAsync(s->f[1], 0, fbytes + 64, s->stream));
-  HIP_TRY(hipMemsetAsync(s->kind, 0, n + 64, s->stream));
+  {
+    hipError_t e = hipMemsetAsync(s->flags, 0, 4 * n, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->sdf, 0, 4 * n, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->f[0], 0, fbytes + 64, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->f[1], 0, fbytes + 64, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->kind, 0, n + 64, s->stream);
+    if (e != hipSuccess) {
+      wt_destroy(s);
+      return fail(WT_E_HIP, std::string("wt_create: ") + hipGetErrorString(e));
+    }
+  }
   std::memset(&s->W, 0, sizeof(s->W));
   if (int r = reclassify(s)) { wt_destroy(s); return r; }
   if (int r = wt_reset(s)) { wt_destroy(s); return r; }
@@ -904,7 +929,11 @@ int wt_autotune(wt_solver* s, int32_t steps, wt_tune_result* out) {
   if (rc == WT_OK) {
     s->item_chunk = out->chunk;
   } else {
+    // a failed trial leaves the handle's launch configuration (and its wall list) as it was
+    const std::string msg = g_err;
     s->item_chunk = chunk0;
+    (void)wt_set_tuning(s, WT_TUNE_CELLS_PER_THREAD, cpt0);
+    g_err = msg;
   }
   // restore the state the trials stepped away from
   const hipError_t e = hipMemcpyAsync(s->f[parity0], keep, fbytes, hipMemcpyDeviceToDevice, s->stream);
@@ -978,8 +1007,11 @@ int wt_batch_create(wt_solver* const* members, int32_t count, wt_batch** out) {
   for (int32_t k = 0; k < count; k++) {
     const wt_solver* s = members[k];
     if (!s) return fail(WT_E_ARG, "null batch member");
-    if (s->q != m0->q || s->prec != m0->prec || s->P.cpt != m0->P.cpt || (s->P.collision != 1) != (m0->P.collision != 1))
+    if (s->q != m0->q || s->prec != m0->prec || cells_per_thread(s->P) != cells_per_thread(m0->P) ||
+        (s->P.collision != 1) != (m0->P.collision != 1))
       return fail(WT_E_ARG, "batch members must share lattice, storage precision, cells per thread and collision");
+    if (s->P.closure == WT_CLOSURE_MLP && s->mlp)
+      return fail(WT_E_STATE, "a batch member uses the learned closure, which the batched step does not run");
     if (s->parity != m0->parity) return fail(WT_E_STATE, "batch members must have been stepped equally often (same parity)");
     if (s->device != m0->device) return fail(WT_E_ARG, "batch members must live on one device");
     for (int32_t j = 0; j < k; j++)
@@ -989,7 +1021,7 @@ int wt_batch_create(wt_solver* const* members, int32_t count, wt_batch** out) {
   b->m.assign(members, members + count);
   b->q = m0->q;
   b->prec = m0->prec;
-  b->cpt = m0->P.cpt;
+  b->cpt = cells_per_thread(m0->P);
   b->reg = m0->P.collision != 1;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {

@@ -156,7 +156,8 @@ class SolverGPU:
     # -- geometry ---------------------------------------------------------------------------
     def uploadFlags(self, flags):
         a = np.ascontiguousarray(flags, dtype=np.uint32).reshape(-1)
-        assert a.size == self.n, "flags must have n entries"
+        if a.size != self.n:
+            raise ValueError("flags must have n = %d entries, got %d" % (self.n, a.size))
         N.check(self._lib.wt_upload_flags(self._h, N.ptr(a)))
 
     def readFlags(self):
@@ -166,7 +167,8 @@ class SolverGPU:
 
     def uploadSdf(self, d):
         a = np.ascontiguousarray(d, dtype=np.float32).reshape(-1)
-        assert a.size == self.n, "sdf must have n entries"
+        if a.size != self.n:
+            raise ValueError("sdf must have n = %d entries, got %d" % (self.n, a.size))
         N.check(self._lib.wt_upload_sdf(self._h, N.ptr(a)))
 
     def readSdf(self):
@@ -179,6 +181,12 @@ class SolverGPU:
         """wheels: list of dicts {c: [x,y,z], axis: [x,y,z], r, hw} in grid space (SolverGPU.ts:325)."""
         a = wheel_array(wheels)
         N.check(self._lib.wt_set_wheels(self._h, N.ptr(a) if len(a) else None, len(a) // 8))
+        self._wheels = [dict(w) for w in list(wheels or [])[:16]]
+
+    @property
+    def wheels_(self):
+        """The wheel list last set (setWheels or voxelizeMesh)."""
+        return list(getattr(self, "_wheels", []))
 
     def voxelizeSdf(self, prims):
         """Analytic-SDF voxelizer (a list of N.Prim); returns VoxelInfo as a dict."""
@@ -196,6 +204,7 @@ class SolverGPU:
         vi = N.VoxelInfo()
         N.check(self._lib.wt_voxelize_mesh(self._h, N.ptr(pos), pos.size // 3, N.ptr(idx), idx.size // 3, N.ptr(m),
                                            N.ptr(wa) if len(wa) else None, len(wa) // 8, N.C.byref(vi)))
+        self._wheels = [dict(w) for w in list(wheels or [])[:16]]
         return voxel_info(vi)
 
     # -- stepping ---------------------------------------------------------------------------
