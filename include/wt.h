@@ -199,6 +199,10 @@ int wt_sync(wt_solver* s);
 
 /* readPopulations(), SolverGPU.ts:359-369, but unpacked to full f_i (q * n floats). */
 int wt_read_populations(wt_solver* s, float* out);
+/* readPopulations() restricted to the planes z0 .. z0 + count - 1 (no reference counterpart: a grid
+ * of 10^8 cells does not fit one host array): full f_i, direction-major within the slab,
+ * out[i * m + c], m = nx * ny * count, c = x + nx * (y + ny * (z - z0)). */
+int wt_read_population_planes(wt_solver* s, int32_t z0, int32_t count, float* out);
 /* No reference counterpart: overwrite the current populations (full f_i, q * n floats) after any
  * pending init; rounds to the storage format. Seeded initial conditions and checkpoints. */
 int wt_upload_populations(wt_solver* s, const float* f);

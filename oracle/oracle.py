@@ -221,6 +221,15 @@ class OracleSolver:
         self._lib.or_read_macro(self._h, _ptr(vel), _ptr(ma), _ptr(mb))
         return vel, ma, mb
 
+    def voxelizeMesh(self, positions, indices, matrix, wheels=()):
+        """The mesh voxelizer's CPU restatement (voxelize_mesh) uploaded into this solver: flags
+        with wheel tags, wall distances and the wheel list; returns the VoxelInfo dict."""
+        flags, sdf_bits, info = voxelize_mesh(positions, indices, matrix, self._dims, wheels)
+        self.setWheels(list(wheels))
+        self.uploadFlags(flags)
+        self.uploadSdf(decode_sdf(sdf_bits).astype(np.float32))
+        return info
+
     def setMlp(self, net, wdist, pair=2):
         """Attach the learned closure: `net` a closure network, `wdist` the wall distance in cells
         (n values, stored as round(32 d) bytes like the product), `pair` the product's cells per

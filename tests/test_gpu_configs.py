@@ -1,0 +1,159 @@
+"""The benchmark configurations at their full sizes on the GPU.
+
+* The sedan LES at 256 x 128 x 128 with FP16 deviation storage (moving belt, rolling wheels, Bouzidi
+  walls, Cs = 0.1, u = 0.08): 100 steps from the impulsive start against the FP64 oracle with FP16
+  store rounding, on the GPU run's own flags, wall distances and wheels. Populations within 2e-3
+  relative (the FP16 bar), force within 5e-3 of its scale.
+* The seeded 24-domain yaw / ride-height sweep (128 x 64 x 64 ... 320 x 160 x 160): the batched run
+  (one launch per step, 100-step hipGraph chunks) equals every domain stepped on its own, bit for bit.
+* The empty tunnel at the headline settings (512 x 256 x 256, FP16, nu = 6e-4, the app's sponge, inlet
+  layer and wall defaults, moving belt) stays inside the reference's free-stream bands
+  (validation.ts:78-81: |max|u| / U - 1| < 1 %, |rho_ref - 1| < 1 %) over 5000 steps.
+* 1024 x 512 x 512 (268 M cells; 5.1e9 population elements, past 2^32): with periodic z, a geometry
+  rolled by k cells along z rolls the populations by k, bit for bit; planes near the top of the
+  domain (element offsets above 2^32) are compared with planes near the bottom.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _sedan_params(precision="f16"):
+    return dict(u=0.08, nu=6e-4, cs=0.1, ground="moving", sides="freeslip", spongeNu=0.04, spongeStart=0.84,
+                spongeIn=0.03, tauWall=0.53, interp=True, collision="regularized", emaAlpha=0.02, precision=precision)
+
+
+def test_sedan_les_256_fp16_matches_oracle_100_steps():
+    from oracle.oracle import OracleSolver
+    from windtunnel_vis_amd import SolverGPU
+    from windtunnel_vis_amd import geometry as G
+    dims = (256, 128, 128)
+    p = _sedan_params()
+    g = SolverGPU(dims, p)
+    body = G.sedan(dims)
+    g.setWheels(body.wheels)
+    info = g.voxelizeSdf(body.prims())
+    print(f"[sedan 256 fp16 vs oracle] solid {info['solidCells']} frontal {info['frontal']}", flush=True)
+    o = OracleSolver(dims, p, storage="f16")
+    o.setWheels(g.wheels_)
+    o.uploadFlags(g.readFlags())
+    o.uploadSdf(g.readSdf())
+    try:
+        sg = g.stepAndSample(100)
+        so = o.stepAndSample(100)
+        a = g.readPopulations().astype(np.float64)
+        b = o.readPopulations()
+        err = np.abs(a - b) / np.abs(b)
+        k = int(np.argmax(err))
+        print("  max rel err %.3e at dir %d cell %d; Fx gpu %.6f oracle %.6f" % (err.max(), k // g.n, k % g.n, sg.fx, so.fx),
+              flush=True)
+        assert err.max() < 2e-3
+        rg, ro = g.sampleRaw(), o.sampleRaw()
+        scale = max(1.0, np.abs(ro[:3]).max())
+        assert np.all(np.abs(rg[:3] - ro[:3]) <= 5e-3 * scale), (rg, ro)
+        assert rg[5] == ro[5] and rg[7] == ro[7] and rg[6] == ro[6] == 0
+        assert not sg.unstable
+    finally:
+        g.destroy()
+        o.destroy()
+
+
+def test_sweep_24_domains_batched_equals_individual():
+    from windtunnel_vis_amd import BatchGPU, sweep
+    cases = sweep.sweep_cases()
+    assert len(cases) == 24
+    A = [m[0] for m in sweep.make_solvers(cases)]
+    B = [m[0] for m in sweep.make_solvers(cases)]
+    cells = sum(s.n for s in A)
+    print(f"[sweep 24 domains] {cells} cells, dims {cases[0]['dims']} ... ", flush=True)
+    try:
+        batch = BatchGPU(A, chunk=100, graph=True)
+        batch.run(250)  # two graph replays + 50 direct launches
+        for s in B:
+            s.step(250)
+        st = batch.stats()
+        for k, (a, b) in enumerate(zip(A, B)):
+            fa, fb = a.readPopulations(), b.readPopulations()
+            assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32)), k
+            del fa, fb
+            sb = b.stepAndSample(0)
+            assert (st[k].fx, st[k].fy, st[k].fz, st[k].step) == (sb.fx, sb.fy, sb.fz, sb.step)
+            assert not st[k].unstable
+        batch.destroy()
+    finally:
+        for s in A + B:
+            s.destroy()
+
+
+@pytest.mark.parametrize("prec", ["f16", "bf16"])
+def test_empty_tunnel_headline_settings_5000_steps(prec):
+    from windtunnel_vis_amd import SolverGPU
+    dims = (512, 256, 256)
+    U = 0.08
+    s = SolverGPU(dims, _sedan_params(prec))
+    assert s.precision == prec
+    s.uploadFlags(np.zeros(s.n, np.uint32))
+    s.reset()
+    try:
+        worst_u = worst_rho = 0.0
+        for k in range(10):
+            st = s.stepAndSample(500)
+            du, dr = st.maxU / U - 1, st.rhoRef - 1
+            worst_u, worst_rho = max(worst_u, abs(du)), max(worst_rho, abs(dr))
+            print(f"  {prec} step {st.step}: maxU/U - 1 {du:+.3e}, rhoRef - 1 {dr:+.3e}", flush=True)
+            assert not st.unstable
+        assert worst_u < 0.01 and worst_rho < 0.01
+    finally:
+        s.destroy()
+
+
+def test_shift_equivariance_1024x512x512():
+    from windtunnel_vis_amd import SolverGPU
+    from windtunnel_vis_amd import _native as N
+    from windtunnel_vis_amd.geometry import prim
+    dims = (1024, 512, 512)
+    nx, ny, nz = dims
+    k = 37
+    p = dict(u=0.1, nu=6e-4, cs=0.1, ground="moving", sides="periodic", precision="f16")
+    a = SolverGPU(dims, p)
+    b = SolverGPU(dims, p)
+    try:
+        # a sphere that reaches the top planes (its cells' element offsets are above 2^32) and a
+        # rounded box on the belt near the bottom
+        a.voxelizeSdf([prim(N.PRIM_SPHERE, (300.0, 200.0, 470.0), radius=45.0),
+                       prim(N.PRIM_ROUNDED_BOX, (600.0, 30.0, 60.0), half=(60.0, 30.0, 40.0), radius=8.0)])
+        flags, sdf = a.readFlags(), a.readSdf()
+        assert (flags.reshape(nz, ny, nx)[500:] != 0).any()
+
+        def roll(v):
+            return np.roll(v.reshape(nz, ny, nx), k, axis=0).reshape(-1)
+
+        b.uploadFlags(roll(flags))
+        b.uploadSdf(roll(sdf))
+        del flags, sdf
+        for s in (a, b):
+            s.reset()
+        print(f"[1024x512x512 equivariance] shift {k}", flush=True)
+        sa, sb = a.stepAndSample(20), b.stepAndSample(20)
+        assert not sa.unstable and not sb.unstable
+        # (planes of a, the planes of b they roll onto): high offsets of a against low offsets of b,
+        # and high offsets of b against middle ones of a
+        for za, cnt in ((500, 12), (440, 8), (0, 4)):
+            zb = (za + k) % nz
+            pa = a.readPopulationPlanes(za, cnt)
+            pb = b.readPopulationPlanes(zb, cnt) if zb + cnt <= nz else None
+            if pb is None:
+                m = nz - zb
+                pb = np.concatenate([b.readPopulationPlanes(zb, m), b.readPopulationPlanes(0, cnt - m)], axis=1)
+            assert np.isfinite(pa).all()
+            mism = int((pa.view(np.uint32) != pb.view(np.uint32)).sum())
+            print(f"  planes {za}+{cnt} of a vs {zb} of b: {mism} mismatches", flush=True)
+            assert mism == 0
+        # the top planes hold flow that changed (not the start state)
+        top = a.readPopulationPlanes(nz - 2, 2)
+        assert float(np.abs(top[0] - top[0, 0]).max()) > 1e-5
+        assert sb.fx == pytest.approx(sa.fx, rel=1e-9, abs=1e-9)
+    finally:
+        a.destroy()
+        b.destroy()

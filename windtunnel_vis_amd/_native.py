@@ -129,6 +129,7 @@ SIGNATURES = {
     "wt_sync": (C.c_int, [_P]),
     "wt_read_populations": (C.c_int, [_P, _P]),
     "wt_upload_populations": (C.c_int, [_P, _P]),
+    "wt_read_population_planes": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
     "wt_read_macro": (C.c_int, [_P, _P, _P, _P]),
     "wt_derive": (C.c_int, [_P]),
     "wt_read_derived": (C.c_int, [_P, _P, _P, _P]),

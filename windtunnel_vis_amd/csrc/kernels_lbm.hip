@@ -935,6 +935,23 @@ __global__ __launch_bounds__(BLOCK) void k_unpack(const typename S::T* __restric
   for (int k = 0; k < L::Q; k++) if (k == i) w = L::w(k);
   out[e] = S::ld(f, d, w);
 }
+// The cells of planes z0 .. z0 + count - 1 as full values, direction-major within the slab:
+// out[i * m + c], m = nx * ny * count, c = x + nx * (y + ny * (z - z0)).
+template <class L, class S>
+__global__ __launch_bounds__(BLOCK) void k_unpack_planes(const typename S::T* __restrict__ f, float* __restrict__ out, KParams P,
+                                                         int32_t z0, int64_t m) {
+  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= (int64_t)L::Q * m) return;
+  const int i = (int)(e / m);
+  const int64_t cell = e - (int64_t)i * m + (int64_t)z0 * P.nx * P.ny;
+  const int64_t row = cell / P.nx;
+  const int x = (int)(cell - row * P.nx);
+  float w = 0.0f;
+#pragma unroll
+  for (int k = 0; k < L::Q; k++) if (k == i) w = L::w(k);
+  out[e] = S::ld(f, pop_at(row, i, x, L::Q, P.nx), w);
+}
+
 template <class L, class S>
 __global__ __launch_bounds__(BLOCK) void k_pack(const float* __restrict__ in, typename S::T* __restrict__ f, KParams P) {
   const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -1140,6 +1157,15 @@ void launch_unpack(const LaunchCtx& c, const void* f, float* out) {
     using L = decltype(l); using S = decltype(s);
     hipLaunchKernelGGL((k_unpack<L, S>), dim3(blocks_for((int64_t)L::Q * c.P.n)), dim3(BLOCK), 0, c.stream,
                        (const typename S::T*)f, out, c.P);
+  });
+}
+
+void launch_unpack_planes(const LaunchCtx& c, const void* f, int32_t z0, int32_t count, float* out) {
+  const int64_t m = (int64_t)c.P.nx * c.P.ny * count;
+  dispatch(c.P.q, c.prec, [&](auto l, auto s) {
+    using L = decltype(l); using S = decltype(s);
+    hipLaunchKernelGGL((k_unpack_planes<L, S>), dim3(blocks_for((int64_t)L::Q * m)), dim3(BLOCK), 0, c.stream,
+                       (const typename S::T*)f, out, c.P, z0, m);
   });
 }
 

@@ -40,6 +40,7 @@ int cells_per_thread(const KParams& P);
 int64_t macro_blocks(int64_t n);
 void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, float* meanB, double* partials, double* result);
 void launch_unpack(const LaunchCtx& c, const void* f, float* out);
+void launch_unpack_planes(const LaunchCtx& c, const void* f, int32_t z0, int32_t count, float* out);
 void launch_pack(const LaunchCtx& c, const float* in, void* f);
 void launch_copy(hipStream_t st, const void* a, void* b, int64_t bytes);
 void launch_clear_stats(const LaunchCtx& c, void* meanA, float* meanB);

@@ -543,6 +543,26 @@ int wt_read_populations(wt_solver* s, float* out) {
   return WT_OK;
 }
 
+int wt_read_population_planes(wt_solver* s, int32_t z0, int32_t count, float* out) {
+  if (!s || !out) return fail(WT_E_ARG, "null argument");
+  if (count < 1 || z0 < 0 || (int64_t)z0 + count > s->dims.nz) return fail(WT_E_ARG, "planes must lie in [0, nz)");
+  if (s->pending_init) {
+    if (int r = run_init(s)) return r;
+  }
+  const int64_t cnt = (int64_t)s->q * s->dims.nx * s->dims.ny * count;
+  float* tmp = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&tmp, 4 * cnt, s->stream));
+  LaunchCtx c = s->ctx();
+  launch_unpack_planes(c, s->f[s->parity], z0, count, tmp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, 4 * cnt, hipMemcpyDeviceToHost, s->stream);
+  const hipError_t e2 = hipFreeAsync(tmp, s->stream);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return fail(WT_E_HIP, std::string("wt_read_population_planes: ") + hipGetErrorString(e));
+  return WT_OK;
+}
+
 int wt_upload_populations(wt_solver* s, const float* f) {
   if (!s || !f) return fail(WT_E_ARG, "null argument");
   if (s->pending_init) {

@@ -235,6 +235,14 @@ class SolverGPU:
         N.check(self._lib.wt_read_populations(self._h, N.ptr(out)))
         return out
 
+    def readPopulationPlanes(self, z0, count):
+        """Populations of the planes z0 .. z0 + count - 1, unpacked, shape (q, nx * ny * count) float32
+        (x fastest, then y, then z - z0)."""
+        m = self.dims["nx"] * self.dims["ny"] * int(count)
+        out = np.empty((self.lattice.q, m), dtype=np.float32)
+        N.check(self._lib.wt_read_population_planes(self._h, int(z0), int(count), N.ptr(out)))
+        return out
+
     def uploadPopulations(self, f):
         a = np.ascontiguousarray(f, dtype=np.float32).reshape(self.lattice.q, self.n)
         N.check(self._lib.wt_upload_populations(self._h, N.ptr(a)))
