@@ -189,6 +189,27 @@ int wt_step(wt_solver* s, int32_t steps);
 /* stepAndSample(steps), SolverGPU.ts:292-309: steps, then macro + momentum exchange + reduction,
  * then a synchronous read of the decoded statistics. steps may be 0. */
 int wt_step_and_sample(wt_solver* s, int32_t steps, wt_stats* out);
+/* stepAndSample(steps) for hosts with an event loop (SolverGPU.ts:292-309: encode and submit, then
+ * await the mapped result): wt_step_and_sample_async enqueues the steps, the macro pass, the
+ * reduction and a copy of it into a pinned slot, and returns a ticket without waiting, so the order
+ * of work on the handle is the order of the calls. wt_sample_wait(ticket) blocks until that copy is
+ * complete and decodes it (the step is the one at the async call); it touches only the ticket's slot
+ * and may run on another thread while other calls use the handle (not wt_destroy). Every ticket
+ * must be waited for once; the slots grow with the samples in flight. The async path does not
+ * change what wt_sample_raw returns. */
+int wt_step_and_sample_async(wt_solver* s, int32_t steps, int64_t* ticket);
+int wt_sample_wait(wt_solver* s, int64_t ticket, wt_stats* out);
+/* encodePost(enc, derived), SolverGPU.ts:225-261: enqueues the macro pass and the force / statistics
+ * reduction on the current populations and, with derived != 0, the derived pass (wt_derive) and an
+ * asynchronous copy of the reduction into a free slot of a ring of four pinned read slots; returns
+ * without waiting. *queued (may be NULL) is 1 when a read was queued, 0 when derived == 0 or every
+ * slot was still busy (that post is then not read back, as in the reference). */
+int wt_encode_post(wt_solver* s, int32_t derived, int32_t* queued);
+/* flushReads(cb), SolverGPU.ts:264-287, as a non-blocking poll: the statistics of every queued read
+ * whose copy has completed, decoded as stepAndSample decodes them, with the step of its post, in
+ * posting order (at most cap of them, into out); their slots are freed. *pending (may be NULL): reads
+ * still in flight. The caller keeps the latest by step and polls again for the pending ones. */
+int wt_flush_reads(wt_solver* s, wt_stats* out, int32_t cap, int32_t* count, int32_t* pending);
 /* The undecoded 8-value reduction of the last sample as the reference's result buffer holds it
  * (Fx, Fy, Fz, max u^2, sum rho_ref, n_ref, nan count, solid count), wgslLbm.ts:491-521. */
 int wt_sample_raw(wt_solver* s, double* out8);

@@ -158,17 +158,34 @@ void mlp_features(const double S[6], double wd, double umag, double nu0, double 
   x[7] = std::log1p(umag * std::max(wd, 0.5) / nu0);
 }
 
+// Distance of v to the nearest BF16 rounding midpoint (where round-to-nearest changes its result).
+double bf16_margin(double v) {
+  const double a = std::fabs(v);
+  if (!(a > 0.0) || !std::isfinite(a)) return INFINITY;
+  int e;
+  std::frexp(a, &e);                       // a in [2^(e-1), 2^e)
+  const double spacing = std::ldexp(1.0, e - 8);  // 8 significant bits
+  const double t = a / spacing;
+  return std::fabs(t - std::floor(t) - 0.5) * spacing;
+}
+
 // The network on normalised inputs with the MFMA path's roundings: BF16 inputs, weights and
 // (after the bias, before the ReLU) first hidden layer; accumulation, second layer and output in
-// FP64 here. Returns nu_t = nu_scale * min(softplus(z), 10).
-double mlp_forward(const or_mlp& net, const double xn[8]) {
+// FP64 here. Returns nu_t = nu_scale * min(softplus(z), 10). margin (may be null): the smallest
+// distance of a BF16-rounded input or positive first-layer value to a rounding midpoint.
+double mlp_forward(const or_mlp& net, const double xn[8], double* margin = nullptr) {
   const double* p = net.params;
   double h1[32], h2[32];
+  double mg = INFINITY;
+  if (margin)
+    for (int i = 0; i < 8; i++) mg = std::min(mg, bf16_margin(xn[i]));
   for (int o = 0; o < 32; o++) {
     double a = p[256 + o];
     for (int i = 0; i < 8; i++) a += round_bf16(p[o * 8 + i]) * round_bf16(xn[i]);
+    if (margin && a > 0.0) mg = std::min(mg, bf16_margin(a));
     h1[o] = std::max(round_bf16(a), 0.0);
   }
+  if (margin) *margin = mg;
   for (int o = 0; o < 32; o++) {
     double a = p[1312 + o];
     for (int i = 0; i < 32; i++) a += round_bf16(p[288 + o * 32 + i]) * h1[i];
@@ -202,6 +219,8 @@ struct Solver {
   bool mlp_on = false;
   or_mlp net;
   std::vector<uint8_t> wdist, use_net;
+  // per cell, the last step's BF16 rounding margin of the network evaluation (infinite off the network)
+  mutable std::vector<double> margin;
 
   // Cells the closure network runs on (see or_mlp): fluid, and no cell of the pair group is solid
   // or has a plain pull (wgslLbm.ts:172) from a solid cell.
@@ -437,7 +456,7 @@ struct Solver {
       double x[8], xn[8];
       mlp_features(S6, wdist[idx] / 32.0, std::sqrt((double)usq), ((double)tau0 - 0.5) / 3.0, x);
       for (int k = 0; k < 8; k++) xn[k] = (x[k] - net.mu[k]) / (net.sigma[k] > 0 ? net.sigma[k] : 1.0);
-      const double nu = mlp_forward(net, xn);
+      const double nu = mlp_forward(net, xn, &margin[idx]);
       if (nu == nu && nu < 1e3) {
         const double nus = ((double)tau - (double)t0) / 3.0;
         tau = t0 + T(3.0 * std::max(nu, net.smag_floor * nus));
@@ -469,6 +488,7 @@ struct Solver {
   void do_step(int steps) {
     if (pending_init) do_init();
     if (mlp_on) classify_net();
+    margin.assign(n, INFINITY);
     const int Q = L.q;
     for (int s = 0; s < steps; s++) {
       const T* fin = cur();
@@ -728,6 +748,15 @@ void or_set_mlp(void* hp, const or_mlp* net, const uint8_t* wdist) {
       s->wdist.assign(wdist, wdist + s->n);
       s->mlp_on = true;
     }
+  });
+}
+
+// Per cell, the BF16 rounding margin of the last step's network evaluation (infinite where the
+// network did not run): where it is below the FP32 error of the device's inputs, the device may
+// round that input the other way.
+void or_mlp_margins(void* hp, double* out) {
+  OR_DISPATCH({
+    for (int64_t i = 0; i < s->n; i++) out[i] = (size_t)i < s->margin.size() ? s->margin[i] : INFINITY;
   });
 }
 

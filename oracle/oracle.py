@@ -83,6 +83,7 @@ def load():
         "or_round_bf16": (C.c_double, [C.c_double]),
         "or_set_mlp": (None, [P, P, P]),
         "or_mlp_infer": (None, [P, P, C.c_int64, P]),
+        "or_mlp_margins": (None, [P, P]),
         "or_mlp_features": (None, [P, P, P, C.c_int64, C.c_double, P]),
     }
     for name, (res, args) in sig.items():
@@ -241,6 +242,15 @@ class OracleSolver:
         self._mlp = mlp_struct(net, pair)
         self._wd = np.ascontiguousarray(np.rint(np.asarray(wdist, np.float64).reshape(-1) * 32), dtype=np.uint8)
         self._lib.or_set_mlp(self._h, C.byref(self._mlp), _ptr(self._wd))
+
+
+    def mlpMargins(self):
+        """Per cell, the BF16 rounding margin of the last step's closure evaluation: the smallest
+        distance of a BF16-rounded network input or positive first-layer value to a rounding
+        midpoint (inf where the network did not run)."""
+        out = np.empty(self.n)
+        self._lib.or_mlp_margins(self._h, _ptr(out))
+        return out
 
 
 def mlp_infer(net, raw):

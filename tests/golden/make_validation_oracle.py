@@ -7,8 +7,10 @@ CI runs them (valscale 0.5, scripts/harness.mjs:11-15):
   through the mesh voxelizer's restatement).
 
 The GPU tests compare the MI355X solver's values with these (the error against the FP64 oracle for
-St and C_D). Run from the repository root:  python tests/golden/make_validation_oracle.py [cases...]
-Takes a few minutes on 8 cores.
+St and C_D). Run from the repository root:
+  python tests/golden/make_validation_oracle.py [--scale=0.5] [cases...]
+Valscale 0.5 takes about five minutes on 8 cores; the Ahmed case at valscale 1 (L = 64, the GPU
+test's size) about an hour and a half on 5.
 """
 import json
 import os
@@ -24,18 +26,23 @@ from windtunnel_vis_amd import validation as V  # noqa: E402
 OUT = os.path.join(ROOT, "tests", "golden", "validation_oracle.json")
 
 
-def main(cases):
-    data = json.load(open(OUT)) if os.path.exists(OUT) else {}
+def main(cases, scale=0.5):
     make = lambda d, p: OracleSolver(d, p)  # noqa: E731
     for name in cases:
         t0 = time.time()
-        r = V.CASES[name](make, scale=0.5)
-        m = {k: (v if not isinstance(v, float) else round(v, 10)) for k, v in r["metrics"].items()}
-        data[name + "_valscale_0.5"] = dict(metrics=m, passed=bool(r["passed"]), oracle="FP64 CPU restatement",
-                                           seconds=round(time.time() - t0, 1))
-        print(name, data[name + "_valscale_0.5"], flush=True)
+        r = V.CASES[name](make, scale=scale)
+        m = {k: (float(v) if isinstance(v, float) else v) for k, v in r["metrics"].items()}
+        key = "%s_valscale_%g" % (name, scale)
+        entry = dict(metrics=m, passed=bool(r["passed"]), oracle="FP64 CPU restatement", seconds=round(time.time() - t0, 1))
+        print(key, entry, flush=True)
+        data = json.load(open(OUT)) if os.path.exists(OUT) else {}
+        data[key] = entry
         json.dump(data, open(OUT, "w"), indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["cylinder", "ahmed"])
+    args = sys.argv[1:]
+    scale = 0.5
+    if args and args[0].startswith("--scale="):
+        scale = float(args.pop(0).split("=", 1)[1])
+    main(args or ["cylinder", "ahmed"], scale)

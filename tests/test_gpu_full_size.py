@@ -51,7 +51,7 @@ def test_shift_equivariance_periodic_z(prec, learned):
             closure.attach(a, net)
             closure.attach(b, net)
         # a sphere and a rounded box off the z centre, the box on the belt (ground contact)
-        a.voxelizeSdf([prim(N.PRIM_SPHERE, (150.0, 120.0, 90.0), radius=30.0),
+        a.voxelizeSdf([prim(N.PRIM_SPHERE, (150.0, 120.0, 90.0), half=(30.0, 0.0, 0.0)),
                        prim(N.PRIM_ROUNDED_BOX, (260.0, 20.0, 170.0), half=(40.0, 22.0, 30.0), radius=6.0)])
         flags, sdf = a.readFlags(), a.readSdf()
         solid = int((flags != 0).sum())

@@ -5,8 +5,17 @@ the moving belt with a contact patch and both sponges (d3_full), and rows of thr
 (d3_long_rows). The network is a seeded random one with smag_floor = 0, so its inputs decide tau
 everywhere off the body: nu_t spans about 0 to 2e-2 against nu = 1e-2.
 
-Tolerances (max relative population error), as for the plain step: FP32 storage 1e-5 against the
-FP64 oracle, FP16 deviation storage 2e-3 against the oracle with FP16 store rounding.
+Tolerances (max relative population error): FP16 deviation storage 2e-3 against the oracle with
+FP16 store rounding, as for the plain step; FP32 storage 5e-5 against the FP64 oracle (the plain
+step's bar is 1e-5). The network's inputs and first hidden layer are rounded to BF16, so a value
+whose FP32 (device) and FP64 (oracle) forms straddle a BF16 rounding midpoint rounds to neighbouring
+BF16 numbers on the two sides: about one cell in a thousand per step, each moving that cell's nu_t by
+a fraction of a percent (the FP32 invariants tr S^3 and det S lose digits to cancellation, so the
+device's inputs can be off by up to about 1e-4 near such a midpoint).
+test_learned_step_single_collision_differs_at_rounding_midpoints checks that picture after a single
+collision: at most 0.5 % of the network cells differ by more than 2e-6 (none by 2e-5), and those
+cells are ones whose inputs lie unusually close to a BF16 midpoint (median distance under 5 % of
+the typical one).
 """
 import numpy as np
 import pytest
@@ -16,7 +25,7 @@ from tests.cases import SEED, lattice_for, make_case, perturbed_state
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f32": 1e-5, "f16": 2e-3}
+TOL = {"f32": 5e-5, "f16": 2e-3}
 ORACLE_STORAGE = {"f32": None, "f16": "f16"}
 
 
@@ -76,7 +85,33 @@ def test_learned_step_matches_oracle_100_steps(name, prec):
         ref.step(100)
         plain = np.abs(ref.readPopulations() - b) / np.abs(b)
         print("  learned vs Smagorinsky (oracle): %.3e" % plain.max(), flush=True)
-        assert plain.max() > 20 * TOL[prec]
+        assert plain.max() > 5 * TOL[prec]
+    finally:
+        g.destroy()
+        o.destroy()
+
+
+@pytest.mark.parametrize("name", ["d3_full", "d3_long_rows"])
+def test_learned_step_single_collision_differs_at_rounding_midpoints(name):
+    net = _net()
+    g, o = _pair(name, "f32", net)
+    try:
+        g.step(1)
+        o.step(1)
+        a, b = g.readPopulations().astype(np.float64), o.readPopulations()
+        err = (np.abs(a - b) / np.abs(b)).max(axis=0)  # per cell
+        margin = o.mlpMargins()
+        net_cells = np.isfinite(margin)
+        off = np.flatnonzero(err > 2e-6)
+        m_all = float(np.median(margin[net_cells]))
+        m_off = float(np.median(margin[off])) if len(off) else 0.0
+        print(f"  {name}: {len(off)} of {int(net_cells.sum())} network cells differ by > 2e-6 (max {err.max():.2e}); "
+              f"median BF16 margin {m_off:.2e} there against {m_all:.2e} over all network cells", flush=True)
+        # isolated cells, small differences
+        assert len(off) <= 0.005 * net_cells.sum() and err.max() < 2e-5
+        # and they are cells whose inputs sit close to a BF16 midpoint: their margins are far below
+        # the typical margin (FP32 differences that come from elsewhere would not pick such cells)
+        assert m_off < 0.05 * m_all, [(int(c), float(err[c]), float(margin[c])) for c in off][:8]
     finally:
         g.destroy()
         o.destroy()

@@ -56,6 +56,61 @@ def test_node_matches_python_host():
     assert t["head"] == head and abs(t["trail"] - float(hist.astype(np.float64).sum())) <= 1e-9 * max(1.0, float(np.abs(hist).sum()))
 
 
+def test_node_async_frames_and_calls_during_a_sample():
+    """encodePost / flushReads (the reference's out-of-order-tolerant frame reads): every delivered
+    read carries its post's step, in order, and the last equals a synchronous sample of the same
+    state. encodeSteps during an un-awaited stepAndSample runs after it (the handle lock): the results
+    equal a strictly sequential run on a second solver. destroy() during a sample is deferred: the
+    sample still resolves and later calls throw."""
+    r = subprocess.run([NODE, "tests/node/async_frames.mjs"], cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o = json.loads(r.stdout)
+    print(o)
+    assert o["burst"][:4] == [10, 20, 30, 40] and o["burst"] == sorted(o["burst"])
+    assert set(o["burst"]) <= {10, 20, 30, 40, 50, 60}
+    steps = [f["step"] for f in o["frames"]]
+    assert steps == o["burst"] + [60]
+    assert o["latest"] == 60 and o["syncStep"] == 60
+    assert o["frames"][-1]["fx"] == o["syncFx"]
+    race = o["race"]
+    assert race["a1"] == race["b1"] and race["a2"] == race["b2"] and race["a2"][0] == 230
+    assert o["destroyed"] == {"step": 300, "unstable": False, "threwAfterDestroy": True}
+
+
+def test_python_host_async_reads_match_sync_sample():
+    from windtunnel_vis_amd import SolverGPU
+    import time
+    dims = (24, 12, 10)
+    s = SolverGPU(dims, dict(u=0.06, nu=0.01, cs=0.1, ground="moving", precision="f16"))
+    got = []
+
+    def drain():
+        t0 = time.time()
+        while True:
+            done, pending = s.flushReads()
+            got.extend(done)
+            if pending == 0 or time.time() - t0 > 10:
+                return pending
+
+    queued = []
+    for k in range(6):
+        s.step(10)
+        queued.append(s.encodePost(True))
+        got += s.flushReads()[0]
+    assert drain() == 0
+    assert queued[:4] == [True] * 4
+    steps = [g.step for g in got]
+    assert steps == [10 * (k + 1) for k in range(6) if queued[k]]
+    assert s.encodePost(True)
+    assert drain() == 0
+    assert got[-1].step == 60
+    st = s.stepAndSample(0)
+    assert (got[-1].fx, got[-1].fy, got[-1].maxU, got[-1].rhoRef) == (st.fx, st.fy, st.maxU, st.rhoRef)
+    assert s.latest.step == 60
+    s.destroy()
+
+
 def test_node_harness_freestream_and_cylinder():
     r = subprocess.run([NODE, "windtunnel_vis_amd/node/harness.mjs", "--case=freestream,cylinder", "--valscale=0.5"],
                        cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)

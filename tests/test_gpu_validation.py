@@ -34,6 +34,45 @@ def test_sphere_drag(prec):
     assert r["passed"]
 
 
+def _oracle_fixture(key):
+    import json
+    import os
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "validation_oracle.json")
+    return json.load(open(p))[key]["metrics"]
+
+
+@pytest.mark.parametrize("prec,tol_st,tol_cd", [("f32", 0.01, 0.005), ("f16", 0.02, 0.01)])
+def test_cylinder_st_and_cd_against_fp64_oracle(prec, tol_st, tol_cd):
+    """The cylinder case at the reference CI's size (valscale 0.5: D = 12, 300 x 120) against the FP64
+    oracle's run of the same case (tests/golden/validation_oracle.json, make_validation_oracle.py):
+    relative error of St and of the mean C_D."""
+    ref = _oracle_fixture("cylinder_valscale_0.5")
+    r = V.cylinder(_make(prec), scale=0.5)
+    m = r["metrics"]
+    e_st = abs(m["St"] - ref["St"]) / ref["St"]
+    e_cd = abs(m["cd"] - ref["cd"]) / ref["cd"]
+    print(f"cylinder {prec}: St {m['St']:.5f} (oracle {ref['St']:.5f}, err {e_st:.2e}), "
+          f"C_D {m['cd']:.5f} (oracle {ref['cd']:.5f}, err {e_cd:.2e}), cycles {m['cycles']}", flush=True)
+    assert r["passed"]
+    assert e_st < tol_st and e_cd < tol_cd
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_ahmed_against_fp64_oracle(prec):
+    """The Ahmed case at the reference CI's size (valscale 0.5, L = 32) against the FP64 oracle's run
+    of it. The oracle (the reference's kernels restated) reads C_D near 0.78 there, far outside the
+    reference's own band (0.285 +- 35 %): the band is not reachable by the reference's method at
+    these resolutions. The GPU run must read the oracle's value within 3 standard errors of the two
+    windows plus 3 %."""
+    ref = _oracle_fixture("ahmed_valscale_0.5")
+    r = V.ahmed(_make(prec), scale=0.5)
+    m = r["metrics"]
+    tol = 3 * (m["se"] ** 2 + ref["se"] ** 2) ** 0.5 + 0.03 * ref["cd"]
+    print(f"ahmed L=32 {prec}: C_D {m['cd']:.4f} +- {m['se']:.4f} (oracle {ref['cd']:.4f} +- {ref['se']:.4f}), tol {tol:.4f}",
+          flush=True)
+    assert abs(m["cd"] - ref["cd"]) < tol
+
+
 @pytest.mark.parametrize("prec", ["f32", "f16"])
 def test_ahmed_drag(prec):
     # The reference's band (0.285 +- 35 %) is not reached at L = 64 cells and Re_sim = 2e4: the

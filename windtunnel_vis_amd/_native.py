@@ -126,6 +126,10 @@ SIGNATURES = {
     "wt_step": (C.c_int, [_P, C.c_int32]),
     "wt_step_and_sample": (C.c_int, [_P, C.c_int32, C.POINTER(Stats)]),
     "wt_sample_raw": (C.c_int, [_P, _P]),
+    "wt_step_and_sample_async": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64)]),
+    "wt_sample_wait": (C.c_int, [_P, C.c_int64, C.POINTER(Stats)]),
+    "wt_encode_post": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    "wt_flush_reads": (C.c_int, [_P, C.POINTER(Stats), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "wt_sync": (C.c_int, [_P]),
     "wt_read_populations": (C.c_int, [_P, _P]),
     "wt_upload_populations": (C.c_int, [_P, _P]),

@@ -1,6 +1,7 @@
 // Private to libwt: the solver handle's state and the helpers the entry-point files share.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,6 +71,23 @@ struct wt_solver {
   wt::MlpDev* mlp = nullptr;        // network in the MFMA layout (device)
   int64_t* pull = nullptr;          // bulk pull table (device), rebuilt with the parameters
   std::vector<int64_t> h_pull;      // its host copy
+  // asynchronous statistics reads (wt_encode_post / wt_flush_reads): a ring of pinned slots, each
+  // filled by a copy on the handle's stream and marked by an event, like the reference's staging
+  // buffers (SolverGPU.ts:86-88, :249-257)
+  struct ReadSlot {
+    double* host = nullptr;  // pinned, 8 doubles
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    int64_t step = 0;
+    uint64_t seq = 0;        // order of the posts
+    uint32_t gen = 0;        // bumped at every reuse (asynchronous samples' tickets)
+  };
+  std::vector<ReadSlot> reads;
+  uint64_t read_seq = 0;
+  // asynchronous stepAndSample (wt_step_and_sample_async / wt_sample_wait): a pool that grows
+  // when every slot is in flight; `read_mu` guards it, since wt_sample_wait may run on another thread
+  std::vector<ReadSlot*> samples;
+  std::mutex read_mu;
 
   wt::LaunchCtx ctx() const {
     wt::LaunchCtx c;
@@ -95,6 +113,7 @@ int reclassify(wt_solver* s);
 int do_steps(wt_solver* s, int steps);
 int do_sample(wt_solver* s);
 void decode_stats(const wt_solver* s, wt_stats* out);
+void decode_result(const double* d, int64_t step, wt_stats* out);
 int64_t storage_bytes(const wt_solver* s);  // bytes of one population buffer
 int ensure_wdist(wt_solver* s);             // wall distances for the learned closure
 }  // namespace wt

@@ -179,16 +179,17 @@ int do_sample(wt_solver* s) {
   return WT_OK;
 }
 
-// SolverGPU.ts:302-308
-void decode_stats(const wt_solver* s, wt_stats* out) {
-  const double* d = s->h_result;
+// SolverGPU.ts:302-308 (and the same decode in flushReads, :270-279)
+void decode_result(const double* d, int64_t step, wt_stats* out) {
   out->fx = d[0]; out->fy = d[1]; out->fz = d[2];
   out->max_u = std::sqrt(std::max(d[3], 0.0));
   out->rho_ref = d[5] > 0 ? d[4] / d[5] : 1.0;
   out->unstable = (d[6] > 0 || !std::isfinite(d[0]) || d[3] > 0.25) ? 1 : 0;
   out->pad = 0;
-  out->step = s->step;
+  out->step = step;
 }
+
+void decode_stats(const wt_solver* s, wt_stats* out) { decode_result(s->h_result, s->step, out); }
 
 int64_t storage_bytes(const wt_solver* s) { return (int64_t)s->q * s->n * (s->prec == WT_PREC_F32 ? 4 : 2); }
 
@@ -379,6 +380,15 @@ int wt_destroy(wt_solver* s) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->h_result) (void)hipHostFree(s->h_result);
+  for (auto& r : s->reads) {
+    if (r.ev) (void)hipEventDestroy(r.ev);
+    if (r.host) (void)hipHostFree(r.host);
+  }
+  for (auto* r : s->samples) {
+    if (r->ev) (void)hipEventDestroy(r->ev);
+    if (r->host) (void)hipHostFree(r->host);
+    delete r;
+  }
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return WT_OK;
@@ -514,6 +524,58 @@ int wt_step_and_sample(wt_solver* s, int32_t steps, wt_stats* out) {
   return WT_OK;
 }
 
+int wt_step_and_sample_async(wt_solver* s, int32_t steps, int64_t* ticket) {
+  if (!s || !ticket) return fail(WT_E_ARG, "null argument");
+  if (steps < 0) return fail(WT_E_ARG, "steps must be >= 0");
+  if (int r = do_steps(s, steps)) return r;  // runs a pending init first
+  LaunchCtx c = s->ctx();
+  launch_macro(c, s->f[s->parity], s->vel, s->meanA, s->meanB, s->partials, s->result);
+  HIP_TRY(hipGetLastError());
+  std::lock_guard<std::mutex> g(s->read_mu);
+  size_t idx = 0;
+  while (idx < s->samples.size() && s->samples[idx]->busy) idx++;
+  if (idx == s->samples.size()) {
+    auto* r = new wt_solver::ReadSlot();
+    if (hipHostMalloc((void**)&r->host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) {
+      if (r->host) (void)hipHostFree(r->host);
+      delete r;
+      return fail(WT_E_NOMEM, "sample slot allocation failed");
+    }
+    s->samples.push_back(r);
+  }
+  wt_solver::ReadSlot* r = s->samples[idx];
+  HIP_TRY(hipMemcpyAsync(r->host, s->result, 8 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipEventRecord(r->ev, s->stream));
+  r->busy = true;
+  r->step = s->step;
+  r->gen++;
+  *ticket = ((int64_t)idx << 32) | (int64_t)r->gen;
+  return WT_OK;
+}
+
+int wt_sample_wait(wt_solver* s, int64_t ticket, wt_stats* out) {
+  if (!s || !out) return fail(WT_E_ARG, "null argument");
+  const size_t idx = (size_t)((uint64_t)ticket >> 32);
+  const uint32_t gen = (uint32_t)((uint64_t)ticket & 0xffffffffu);
+  hipEvent_t ev = nullptr;
+  wt_solver::ReadSlot* r = nullptr;
+  {
+    std::lock_guard<std::mutex> g(s->read_mu);
+    if (idx >= s->samples.size() || !s->samples[idx]->busy || s->samples[idx]->gen != gen)
+      return fail(WT_E_ARG, "unknown or already collected sample ticket");
+    r = s->samples[idx];
+    ev = r->ev;
+  }
+  // the wait touches only the ticket's slot: other calls on the handle may run meanwhile
+  const hipError_t e = hipEventSynchronize(ev);
+  std::lock_guard<std::mutex> g(s->read_mu);
+  if (e == hipSuccess) decode_result(r->host, r->step, out);
+  r->busy = false;
+  if (e != hipSuccess) return fail(WT_E_HIP, std::string("wt_sample_wait: ") + hipGetErrorString(e));
+  return WT_OK;
+}
+
 int wt_sample_raw(wt_solver* s, double* out8) {
   if (!s || !out8) return fail(WT_E_ARG, "null argument");
   std::memcpy(out8, s->h_result, 8 * sizeof(double));
@@ -628,6 +690,64 @@ int wt_derive(wt_solver* s) {
   if (int r = wt::ensure_derived(s)) return r;
   wt::launch_derived(s->ctx(), s->vel, s->meanA, s->meanB, s->vort, s->stat, s->meanh);
   HIP_TRY(hipGetLastError());
+  return WT_OK;
+}
+
+int wt_encode_post(wt_solver* s, int32_t derived, int32_t* queued) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (queued) *queued = 0;
+  if (s->pending_init) {
+    if (int r = run_init(s)) return r;
+  }
+  if (derived) {
+    if (int r = wt::ensure_derived(s)) return r;
+  }
+  LaunchCtx c = s->ctx();
+  launch_macro(c, s->f[s->parity], s->vel, s->meanA, s->meanB, s->partials, s->result);
+  if (derived) launch_derived(c, s->vel, s->meanA, s->meanB, s->vort, s->stat, s->meanh);
+  HIP_TRY(hipGetLastError());
+  if (!derived) return WT_OK;  // the reference reads back only with the derived pass (SolverGPU.ts:247)
+  if (s->reads.empty()) {
+    s->reads.resize(4);
+    for (auto& r : s->reads) {
+      if (hipHostMalloc((void**)&r.host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(WT_E_NOMEM, "pinned read slot allocation failed");
+      std::memset(r.host, 0, 8 * sizeof(double));
+      HIP_TRY(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+    }
+  }
+  for (auto& r : s->reads) {
+    if (r.busy) continue;
+    HIP_TRY(hipMemcpyAsync(r.host, s->result, 8 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipEventRecord(r.ev, s->stream));
+    r.busy = true;
+    r.step = s->step;
+    r.seq = ++s->read_seq;
+    if (queued) *queued = 1;
+    return WT_OK;
+  }
+  return WT_OK;  // every slot busy: this post is not read back (SolverGPU.ts:249-250)
+}
+
+int wt_flush_reads(wt_solver* s, wt_stats* out, int32_t cap, int32_t* count, int32_t* pending) {
+  if (!s || !count || (cap > 0 && !out) || cap < 0) return fail(WT_E_ARG, "bad flush arguments");
+  *count = 0;
+  // completed reads in the order they were posted (one stream: they complete in that order)
+  std::vector<wt_solver::ReadSlot*> order;
+  for (auto& r : s->reads)
+    if (r.busy) order.push_back(&r);
+  std::sort(order.begin(), order.end(), [](const wt_solver::ReadSlot* a, const wt_solver::ReadSlot* b) { return a->seq < b->seq; });
+  int32_t left = 0;
+  for (auto* r : order) {
+    if (*count >= cap) { left++; continue; }
+    const hipError_t q = hipEventQuery(r->ev);
+    if (q == hipErrorNotReady) { left++; continue; }
+    if (q != hipSuccess) return fail(WT_E_HIP, std::string("wt_flush_reads: ") + hipGetErrorString(q));
+    decode_result(r->host, r->step, &out[*count]);
+    (*count)++;
+    r->busy = false;
+  }
+  if (pending) *pending = left;
   return WT_OK;
 }
 

@@ -69,6 +69,10 @@ export declare class SolverGPU implements LbmSolver {
   clearStats(): void;
   encodeSteps(steps: number): void;
   stepAndSample(steps: number): Promise<SolverStats>;
+  /** Macro, reduction and (derived) derived pass; queues a statistics read (SolverGPU.ts:225-261). */
+  encodePost(derived?: boolean): void;
+  /** Delivers the queued reads to onStats as they complete, keeping `latest` (SolverGPU.ts:264-287). */
+  flushReads(onStats?: (s: SolverStats) => void): void;
   uploadFlags(flags: Uint32Array): void;
   readFlags(): Promise<Uint32Array>;
   uploadSdf(d: Float32Array): void;

@@ -71,7 +71,8 @@ export class SolverGPU {
     /** populations stored as packed half-width deviation pairs */
     this.packed = this.precision !== 'f32';
     this.latest = null;
-    this.queue = Promise.resolve();
+    this.pendingReads = [];
+    this.pollTimer = null;
   }
 
   get stepCount() { return native.info(this.h).step; }
@@ -92,13 +93,47 @@ export class SolverGPU {
 
   /** Run `steps` steps, then read the force / statistics result (SolverGPU.ts:292-309). */
   stepAndSample(steps) {
-    // one handle is not re-entrant: serialise the async work
-    const run = this.queue.then(() => native.stepAndSample(this.h, steps));
-    this.queue = run.then(() => undefined, () => undefined);
-    return run.then((s) => {
+    // the steps, reduction and read are enqueued now (call order is the GPU order, as the
+    // reference's submit); only the wait for the read runs off the JS thread
+    return native.stepAndSample(this.h, steps).then((s) => {
       if (!this.latest || s.step >= this.latest.step) this.latest = s;
       return s;
     });
+  }
+
+  /**
+   * Macro fields, statistics and force reduction (SolverGPU.ts:225-261): with `derived` the derived
+   * pass runs too and a read of the statistics is queued (up to four in flight; a post that finds
+   * them all busy is not read back, as in the reference). Returns at once.
+   */
+  encodePost(derived = true) {
+    if (native.encodePost(this.h, !!derived)) this.pendingReads.push({ cb: null });
+  }
+
+  /**
+   * Call after encodePost (SolverGPU.ts:264-287): the reads queued since the last flush deliver
+   * their statistics to `onStats` as they complete, in any later turn of the event loop, and
+   * `latest` keeps the newest by step. Never blocks: completed reads are polled.
+   */
+  flushReads(onStats) {
+    for (const r of this.pendingReads) if (!r.cb) r.cb = onStats || (() => {});
+    this._pollReads();
+  }
+
+  _pollReads() {
+    if (this.pollTimer || !this.h || this.pendingReads.length === 0) return;
+    const poll = () => {
+      this.pollTimer = null;
+      if (!this.h) return;
+      const { stats } = native.flushReads(this.h);
+      for (const s of stats) {
+        const r = this.pendingReads.shift();  // reads complete in posting order
+        if (!this.latest || s.step >= this.latest.step) this.latest = s;
+        if (r && r.cb) r.cb(s);
+      }
+      if (this.pendingReads.some((r) => r.cb)) this.pollTimer = setTimeout(poll, 1);
+    };
+    this.pollTimer = setTimeout(poll, 0);
   }
 
   uploadFlags(flags) { native.uploadFlags(this.h, flags); }
@@ -154,7 +189,14 @@ export class SolverGPU {
     return native.streamlines(this.h, rakeArray(rake), seeds, perLine, points, h, !!useMean);
   }
 
+  /**
+   * destroy(), SolverGPU.ts:432-437. Safe while a stepAndSample is in flight: the addon frees the
+   * solver when that sample completes (its promise still settles); pending reads are dropped.
+   */
   destroy() {
+    if (this.pollTimer) clearTimeout(this.pollTimer);
+    this.pollTimer = null;
+    this.pendingReads = [];
     if (this.h) native.destroy(this.h);
     this.h = null;
   }

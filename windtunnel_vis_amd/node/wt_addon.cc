@@ -5,6 +5,7 @@
 #include <node_api.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,8 +27,17 @@ napi_value throw_wt(napi_env env, int rc) {
   return nullptr;
 }
 
+// One solver handle. libwt handles are not re-entrant, so every call on one runs on the JS thread,
+// in call order: stepAndSample enqueues its steps, reduction and read there
+// (wt_step_and_sample_async) and leaves only the wait for the read to a worker thread
+// (wt_sample_wait, which touches nothing but the read's slot). A destroy() while samples are in
+// flight only marks the handle (`closing`); the last sample's completion destroys it. Each
+// in-flight sample holds a reference to the JS handle, so the finalizer cannot run under it.
 struct Handle {
   wt_solver* s = nullptr;
+  std::mutex mu;         // held by the JS-thread calls and by the completion that destroys
+  int pending = 0;       // samples in flight (changed on the JS thread only)
+  bool closing = false;  // destroy() called while samples were in flight
 };
 
 void finalize_handle(napi_env, void* data, void*) {
@@ -43,12 +53,22 @@ Handle* get_handle(napi_env env, napi_value v) {
     return nullptr;
   }
   auto* h = static_cast<Handle*>(p);
-  if (!h->s) {
+  if (!h->s || h->closing) {
     napi_throw_error(env, nullptr, "solver was destroyed");
     return nullptr;
   }
   return h;
 }
+
+// A solver handle with its lock held for the scope (see Handle).
+struct HandleLock {
+  Handle* h = nullptr;
+  std::unique_lock<std::mutex> lk;
+  HandleLock(napi_env env, napi_value v) : h(get_handle(env, v)) {
+    if (h) lk = std::unique_lock<std::mutex>(h->mu);
+  }
+  explicit operator bool() const { return h != nullptr; }
+};
 
 bool get_prop_num(napi_env env, napi_value obj, const char* key, double* out) {
   bool has = false;
@@ -190,9 +210,14 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
   void* p = nullptr;
   napi_get_value_external(env, argv[0], &p);
   auto* h = static_cast<Handle*>(p);
-  if (h && h->s) {
-    wt_destroy(h->s);
-    h->s = nullptr;
+  if (h && h->s && !h->closing) {
+    if (h->pending > 0) {
+      h->closing = true;  // the last in-flight sample destroys it
+    } else {
+      std::lock_guard<std::mutex> g(h->mu);
+      wt_destroy(h->s);
+      h->s = nullptr;
+    }
   }
   return nullptr;
 }
@@ -200,8 +225,9 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
 napi_value SetParams(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   wt_params p{};
   wt_get_params(h->s, &p);
   const uint32_t mask = parse_params(env, argv[1], &p);
@@ -213,8 +239,9 @@ napi_value SetParams(napi_env env, napi_callback_info info) {
 napi_value Simple(napi_env env, napi_callback_info info, int (*fn)(wt_solver*)) {
   napi_value argv[1];
   if (!args(env, info, 1, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   int rc = fn(h->s);
   if (rc) return throw_wt(env, rc);
   return nullptr;
@@ -226,8 +253,9 @@ napi_value Sync(napi_env env, napi_callback_info info) { return Simple(env, info
 napi_value Info(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!args(env, info, 1, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   wt_info i{};
   int rc = wt_info_get(h->s, &i);
   if (rc) return throw_wt(env, rc);
@@ -249,8 +277,9 @@ napi_value Info(napi_env env, napi_callback_info info) {
 napi_value UploadFlags(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   uint32_t* d;
   size_t len;
   if (!typed_array(env, argv[1], napi_uint32_array, &d, &len)) {
@@ -271,8 +300,9 @@ napi_value UploadFlags(napi_env env, napi_callback_info info) {
 napi_value ReadFlags(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!args(env, info, 1, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   wt_info i{};
   wt_info_get(h->s, &i);
   void* data;
@@ -285,8 +315,9 @@ napi_value ReadFlags(napi_env env, napi_callback_info info) {
 napi_value UploadSdf(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   float* d;
   size_t len;
   if (!typed_array(env, argv[1], napi_float32_array, &d, &len)) {
@@ -308,8 +339,9 @@ napi_value UploadSdf(napi_env env, napi_callback_info info) {
 napi_value SetWheels(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   float* d;
   size_t len;
   if (!typed_array(env, argv[1], napi_float32_array, &d, &len) || len % 8 != 0) {
@@ -324,8 +356,9 @@ napi_value SetWheels(napi_env env, napi_callback_info info) {
 napi_value Step(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   int32_t n = 0;
   napi_get_value_int32(env, argv[1], &n);
   int rc = wt_step(h->s, n);
@@ -336,8 +369,10 @@ napi_value Step(napi_env env, napi_callback_info info) {
 struct SampleWork {
   napi_async_work work;
   napi_deferred deferred;
-  wt_solver* s;
-  int32_t steps;
+  Handle* h;
+  napi_ref ref;  // keeps the JS handle (and with it the Handle) alive while the work is in flight
+  wt_solver* s;  // the solver at post time (freed only after the last completion)
+  int64_t ticket;
   int rc;
   std::string err;
   wt_stats st;
@@ -345,12 +380,19 @@ struct SampleWork {
 
 void sample_execute(napi_env, void* data) {
   auto* w = static_cast<SampleWork*>(data);
-  w->rc = wt_step_and_sample(w->s, w->steps, &w->st);
+  w->rc = wt_sample_wait(w->s, w->ticket, &w->st);
   if (w->rc) w->err = wt_last_error();
 }
 
 void sample_complete(napi_env env, napi_status, void* data) {
   auto* w = static_cast<SampleWork*>(data);
+  Handle* h = w->h;
+  if (--h->pending == 0 && h->closing && h->s) {
+    std::lock_guard<std::mutex> g(h->mu);
+    wt_destroy(h->s);
+    h->s = nullptr;
+  }
+  napi_delete_reference(env, w->ref);
   if (w->rc) {
     napi_value msg, err;
     std::string m = "libwt error " + std::to_string(w->rc) + ": " + w->err;
@@ -368,15 +410,24 @@ void sample_complete(napi_env env, napi_status, void* data) {
 napi_value StepAndSample(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
+  int32_t steps = 0;
+  napi_get_value_int32(env, argv[1], &steps);
+  int64_t ticket = 0;
+  int rc = wt_step_and_sample_async(h->s, steps, &ticket);  // enqueued now, in call order
+  if (rc) return throw_wt(env, rc);
   auto* w = new SampleWork();
+  w->h = h;
   w->s = h->s;
-  napi_get_value_int32(env, argv[1], &w->steps);
+  w->ticket = ticket;
   napi_value promise, name;
   NAPI_CALL(env, napi_create_promise(env, &w->deferred, &promise));
+  NAPI_CALL(env, napi_create_reference(env, argv[0], 1, &w->ref));
   napi_create_string_utf8(env, "wt_step_and_sample", NAPI_AUTO_LENGTH, &name);
   NAPI_CALL(env, napi_create_async_work(env, nullptr, name, sample_execute, sample_complete, w, &w->work));
+  h->pending++;
   NAPI_CALL(env, napi_queue_async_work(env, w->work));
   return promise;
 }
@@ -385,8 +436,9 @@ napi_value StepAndSample(napi_env env, napi_callback_info info) {
 napi_value StepAndSampleSync(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   int32_t n = 0;
   napi_get_value_int32(env, argv[1], &n);
   wt_stats st{};
@@ -398,8 +450,9 @@ napi_value StepAndSampleSync(napi_env env, napi_callback_info info) {
 napi_value ReadPopulations(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!args(env, info, 1, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   wt_info i{};
   wt_info_get(h->s, &i);
   void* data;
@@ -412,8 +465,9 @@ napi_value ReadPopulations(napi_env env, napi_callback_info info) {
 napi_value UploadPopulations(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   float* d;
   size_t len;
   wt_info i{};
@@ -431,8 +485,9 @@ napi_value UploadPopulations(napi_env env, napi_callback_info info) {
 napi_value VoxelizeSdf(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   float* d;
   size_t len;
   if (!typed_array(env, argv[1], napi_float32_array, &d, &len) || len % 20 != 0) {
@@ -461,8 +516,9 @@ napi_value VoxelizeSdf(napi_env env, napi_callback_info info) {
 napi_value VoxelizeMesh(napi_env env, napi_callback_info info) {
   napi_value argv[5];
   if (!args(env, info, 5, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   float *pos, *m, *wh;
   uint32_t* idx;
   size_t np, ni, nm, nw;
@@ -482,8 +538,9 @@ napi_value VoxelizeMesh(napi_env env, napi_callback_info info) {
 napi_value Derive(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!args(env, info, 1, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   int rc = wt_derive(h->s);
   if (rc) return throw_wt(env, rc);
   return nullptr;
@@ -493,8 +550,9 @@ napi_value Derive(napi_env env, napi_callback_info info) {
 napi_value ExportFields(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   int32_t stride = 0;
   napi_get_value_int32(env, argv[1], &stride);
   int32_t o[3];
@@ -530,8 +588,9 @@ napi_value ExportFields(napi_env env, napi_callback_info info) {
 napi_value Probe(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   float* pos;
   size_t np;
   if (!typed_array(env, argv[1], napi_float32_array, &pos, &np) || np % 4) {
@@ -566,8 +625,9 @@ bool get_rake(napi_env env, napi_value v, wt_rake* r) {
 napi_value Streamlines(napi_env env, napi_callback_info info) {
   napi_value argv[7];
   if (!args(env, info, 7, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   wt_rake r{};
   if (!get_rake(env, argv[1], &r)) {
     napi_throw_type_error(env, nullptr, "streamlines: rake must be a Float32Array(12): a, b, d as xyzw");
@@ -623,7 +683,7 @@ PHandle* get_particles(napi_env env, napi_value v) {
     napi_throw_error(env, nullptr, "particle system was destroyed");
     return nullptr;
   }
-  if (!ph->owner || !ph->owner->s) {
+  if (!ph->owner || !ph->owner->s || ph->owner->closing) {
     napi_throw_error(env, nullptr, "the particle system's solver was destroyed");
     return nullptr;
   }
@@ -634,8 +694,9 @@ PHandle* get_particles(napi_env env, napi_value v) {
 napi_value ParticlesCreate(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
-  Handle* h = get_handle(env, argv[0]);
-  if (!h) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  Handle* h = lk.h;
   wt_particle_opts o{};
   double v = 0;
   o.count = get_prop_num(env, argv[1], "count", &v) ? (int32_t)v : 0;
@@ -687,6 +748,7 @@ napi_value ParticlesUpdate(napi_env env, napi_callback_info info) {
   if (!args(env, info, 5, argv)) return nullptr;
   PHandle* ph = get_particles(env, argv[0]);
   if (!ph) return nullptr;
+  std::lock_guard<std::mutex> g(ph->owner->mu);  // the tracers run on the solver's stream and fields
   double steps = 0;
   uint32_t seed = 0;
   bool fill = false, advance = true;
@@ -705,6 +767,7 @@ napi_value ParticlesRead(napi_env env, napi_callback_info info) {
   if (!args(env, info, 1, argv)) return nullptr;
   PHandle* ph = get_particles(env, argv[0]);
   if (!ph) return nullptr;
+  std::lock_guard<std::mutex> g(ph->owner->mu);  // the tracers run on the solver's stream and fields
   void *ds, *dh;
   napi_value st = make_typed(env, napi_float32_array, (size_t)ph->count * 4, 4, &ds);
   napi_value tr = make_typed(env, napi_float32_array, (size_t)ph->trail * ph->count * 4, 4, &dh);
@@ -721,6 +784,42 @@ napi_value ParticlesRead(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, out, "state", st);
   napi_set_named_property(env, out, "trail", tr);
   napi_set_named_property(env, out, "head", hv);
+  return out;
+}
+
+// encodePost(h, derived) -> true when a statistics read was queued (SolverGPU.ts:225-261)
+napi_value EncodePost(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  bool derived = true;
+  napi_get_value_bool(env, argv[1], &derived);
+  int32_t queued = 0;
+  int rc = wt_encode_post(lk.h->s, derived ? 1 : 0, &queued);
+  if (rc) return throw_wt(env, rc);
+  napi_value out;
+  napi_get_boolean(env, queued != 0, &out);
+  return out;
+}
+
+// flushReads(h) -> {stats: [SolverStats of the completed reads, in posting order], pending}; never blocks
+napi_value FlushReads(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  wt_stats st[8];
+  int32_t count = 0, pending = 0;
+  int rc = wt_flush_reads(lk.h->s, st, 8, &count, &pending);
+  if (rc) return throw_wt(env, rc);
+  napi_value out, arr, pv;
+  napi_create_object(env, &out);
+  napi_create_array_with_length(env, (size_t)count, &arr);
+  for (int32_t k = 0; k < count; k++) napi_set_element(env, arr, (uint32_t)k, stats_object(env, st[k]));
+  napi_create_int32(env, pending, &pv);
+  napi_set_named_property(env, out, "stats", arr);
+  napi_set_named_property(env, out, "pending", pv);
   return out;
 }
 
@@ -748,6 +847,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"derive", Derive}, {"exportFields", ExportFields}, {"probe", Probe}, {"streamlines", Streamlines},
       {"particlesCreate", ParticlesCreate}, {"particlesDestroy", ParticlesDestroy},
       {"particlesUpdate", ParticlesUpdate}, {"particlesRead", ParticlesRead},
+      {"encodePost", EncodePost}, {"flushReads", FlushReads},
   };
   for (auto& f : fns) {
     napi_value fn;

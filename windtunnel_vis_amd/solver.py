@@ -220,6 +220,28 @@ class SolverGPU:
         N.check(self._lib.wt_step_and_sample(self._h, int(steps), N.C.byref(st)))
         return SolverStats(st.fx, st.fy, st.fz, st.max_u, st.rho_ref, bool(st.unstable), int(st.step))
 
+    def encodePost(self, derived=True):
+        """Macro pass, force reduction and (derived) the derived pass, with an asynchronous read of the
+        statistics queued (SolverGPU.ts:225-261). Returns True when a read was queued."""
+        q = N.C.c_int32()
+        N.check(self._lib.wt_encode_post(self._h, 1 if derived else 0, N.C.byref(q)))
+        return bool(q.value)
+
+    def flushReads(self, onStats=None):
+        """Non-blocking: the statistics of the queued reads that have completed, in posting order
+        (SolverGPU.ts:264-287); each goes to onStats and the latest by step is kept in self.latest.
+        Returns (list of SolverStats, reads still pending)."""
+        arr = (N.Stats * 4)()
+        cnt, pend = N.C.c_int32(), N.C.c_int32()
+        N.check(self._lib.wt_flush_reads(self._h, arr, 4, N.C.byref(cnt), N.C.byref(pend)))
+        out = [SolverStats(a.fx, a.fy, a.fz, a.max_u, a.rho_ref, bool(a.unstable), int(a.step)) for a in arr[:cnt.value]]
+        for s in out:
+            if getattr(self, "latest", None) is None or s.step >= self.latest.step:
+                self.latest = s
+            if onStats:
+                onStats(s)
+        return out, pend.value
+
     def sampleRaw(self):
         out = np.zeros(8, dtype=np.float64)
         N.check(self._lib.wt_sample_raw(self._h, N.ptr(out)))
