@@ -380,6 +380,16 @@ int wt_mlp_load(wt_solver* s, const wt_mlp* net);
 int wt_read_wall_distance(wt_solver* s, uint8_t* out);
 /* nu_t of n cells from their raw (unnormalised) inputs (n x 8), on the step kernel's MFMA path. */
 int wt_mlp_infer(const wt_mlp* net, const float* raw, int64_t n, float* nu);
+/* No reference counterpart. wt_mlp_infer for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, hosts that forbid device allocation): the same arguments, results
+ * and kernels, but the network and the staged inputs and results live in a static device array of
+ * the library (2.3 MB per GPU, held until the process ends) instead of memory allocated per call,
+ * and the work runs on the current device's null stream. No hipMalloc, hipFree or stream creation.
+ * At most WT_MLP_NOALLOC_MAX cells per call (more is WT_E_ARG). Calls on one GPU share that
+ * array: calls that overlap in time on one GPU (other threads) race; each call synchronises
+ * before it returns. */
+#define WT_MLP_NOALLOC_MAX 65536
+int wt_mlp_infer_noalloc(const wt_mlp* net, const float* raw, int64_t n, float* nu);
 
 /* Training data: records of 10 floats (8 raw inputs, target nu_t, coarse cell index as bits) in
  * device memory. wt_dataset_add box-filters a fine D3Q19 run 2x onto the grid of `coarse` (same

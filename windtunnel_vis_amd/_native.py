@@ -161,6 +161,7 @@ SIGNATURES = {
     "wt_mlp_load": (C.c_int, [_P, C.POINTER(Mlp)]),
     "wt_read_wall_distance": (C.c_int, [_P, C.c_void_p]),
     "wt_mlp_infer": (C.c_int, [C.POINTER(Mlp), C.c_void_p, C.c_int64, C.c_void_p]),
+    "wt_mlp_infer_noalloc": (C.c_int, [C.POINTER(Mlp), C.c_void_p, C.c_int64, C.c_void_p]),
     "wt_dataset_create": (C.c_int, [C.c_int64, C.POINTER(_P)]),
     "wt_dataset_destroy": (C.c_int, [_P]),
     "wt_dataset_add": (C.c_int, [_P, _P, _P, C.POINTER(C.c_int64)]),

@@ -5,11 +5,44 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <vector>
 
 #include "solver.hpp"
+
+// Device scratch of wt_mlp_infer_noalloc, which calls no allocator: the network in the MFMA layout,
+// the staged raw inputs and the results of at most WT_MLP_NOALLOC_MAX cells. One copy per device
+// (a __device__ array exists once on every GPU the library runs on, for the life of the process).
+namespace {
+constexpr size_t NOALLOC_NET_BYTES = (sizeof(wt::MlpDev) + 255) / 256 * 256;
+constexpr size_t NOALLOC_RAW_BYTES = sizeof(float) * wt::MLP_IN * (size_t)WT_MLP_NOALLOC_MAX;
+constexpr size_t NOALLOC_NU_BYTES = sizeof(float) * (size_t)WT_MLP_NOALLOC_MAX;
+constexpr size_t NOALLOC_BYTES = NOALLOC_NET_BYTES + NOALLOC_RAW_BYTES + NOALLOC_NU_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// The scratch array's device address, per device, looked up once with that device current. The
+// array's name is only a host-side stand-in for the symbol: host code uses this address.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int scratch_for_current_device(unsigned char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return wt::fail(WT_E_NODEV, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return wt::fail(WT_E_HIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return WT_OK;
+}
+}  // namespace
 
 using namespace wt;
 
@@ -131,6 +164,33 @@ int wt_mlp_infer(const wt_mlp* net, const float* raw, int64_t n, float* nu) {
   if (draw) (void)hipFree(draw);
   if (dnu) (void)hipFree(dnu);
   return rc;
+}
+
+int wt_mlp_infer_noalloc(const wt_mlp* net, const float* raw, int64_t n, float* nu) {
+  if (!net || (n > 0 && (!raw || !nu)) || n < 0) return fail(WT_E_ARG, "bad inference arguments");
+  if (n > WT_MLP_NOALLOC_MAX)
+    return fail(WT_E_ARG, "wt_mlp_infer_noalloc: at most WT_MLP_NOALLOC_MAX (65536) cells per call; split the input or use wt_mlp_infer");
+  if (n == 0) return WT_OK;
+  unsigned char* base = nullptr;
+  if (int r = scratch_for_current_device(&base)) return r;
+  MlpDev h;
+  build_dev(net, &h);
+  MlpDev* dnet = (MlpDev*)base;
+  float* draw = (float*)(base + NOALLOC_NET_BYTES);
+  float* dnu = (float*)(base + NOALLOC_NET_BYTES + NOALLOC_RAW_BYTES);
+  // the null stream, synchronised before returning: the staged inputs and results are in the
+  // shared scratch, so the next call (or another thread's) may overwrite them afterwards
+  hipStream_t st = nullptr;
+  hipError_t e = hipMemcpyAsync(dnet, &h, sizeof(MlpDev), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(draw, raw, sizeof(float) * MLP_IN * (size_t)n, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    launch_mlp_infer(st, dnet, draw, n, dnu);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(nu, dnu, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail(WT_E_HIP, std::string("inference: ") + hipGetErrorString(e));
+  return WT_OK;
 }
 
 int wt_dataset_create(int64_t capacity, wt_dataset** out) {
