@@ -7,6 +7,64 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+def test_controller_driven_run_matches_oracle():
+    """The run controller on the GPU solver and on the FP64 oracle, frame by frame: every frame sets
+    u, nu (start-up ramp and damping), Cs, the floor, the EMA weight and interp through setParams
+    (app.ts:792-810 -> SolverGPU.ts:155-187; the GPU rebuilds its pull table and wall links each
+    time), and mid-run the interpolated walls are switched off and Cs raised. FP32 storage: the
+    statistics of every frame and the final populations agree with the oracle's (1e-5 relative)."""
+    from oracle.oracle import OracleSolver
+    from tests.cases import box_body_3d, make_case
+    from windtunnel_vis_amd import SolverGPU
+    from windtunnel_vis_amd.controller import RunController
+    import numpy as np
+    dims = (32, 16, 12)
+    flags, sdf, wheels = box_body_3d(dims)
+    _, params, _, _, _ = make_case("d3_full")
+    p = dict(params, precision="f32")
+    runs = []
+    for make in (lambda: SolverGPU(dims, p), lambda: OracleSolver(dims, p)):
+        s = make()
+        s.setWheels(wheels)
+        s.uploadFlags(flags)
+        s.uploadSdf(sdf)
+        c = RunController(s, Lcells=8, frontal=20, speed_kmh=100, reOverride=200, ground="moving")
+        hist = []
+        for f in range(60):
+            if f == 30:
+                c.interp = False
+                c.flowChanged()
+            if f == 45:
+                c.cs = 0.15
+                c.flowChanged()
+            st = c.frame(5)
+            hist.append((st.step, st.fx, st.fy, st.maxU, st.rhoRef))
+        runs.append((s, c, np.array(hist)))
+    (g, cg, hg), (o, co, ho) = runs
+    try:
+        assert np.array_equal(hg[:, 0], ho[:, 0])
+        scale = max(1e-3, float(np.abs(ho[:, 1:3]).max()))
+        err_f = float(np.abs(hg[:, 1:3] - ho[:, 1:3]).max()) / scale
+        err_u = float(np.abs(hg[:, 3:] - ho[:, 3:]).max())
+        fg, fo = g.readPopulations().astype(np.float64), o.readPopulations()
+        err_p = float((np.abs(fg - fo) / np.abs(fo)).max())
+        print(f"controller-driven run: force {err_f:.2e}, maxU/rhoRef {err_u:.2e}, populations {err_p:.2e}; "
+              f"cd gpu {cg.readout()['cd']:.6f} oracle {co.readout()['cd']:.6f}", flush=True)
+        assert err_f < 1e-4 and err_u < 1e-6 and err_p < 1e-5
+        assert cg.stab == co.stab
+        # the controllers' readouts (averaging windows, EMA, convergence) follow the same samples
+        rg, ro = cg.readout(), co.readout()
+        for k in ("cd", "cl", "cdSE", "converged"):
+            a, b = rg[k], ro[k]
+            if isinstance(a, float) and math.isnan(a):
+                assert math.isnan(b), k
+            else:
+                assert a == pytest.approx(b, rel=1e-4, abs=1e-6), k
+    finally:
+        g.destroy()
+        o.destroy()
+
+
 def test_controller_runs_sphere():
     from windtunnel_vis_amd import SolverGPU
     from windtunnel_vis_amd import geometry as G

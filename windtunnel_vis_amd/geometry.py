@@ -329,8 +329,13 @@ def normalize_mesh(mesh, align_longest=True):
 
 
 def placement_matrix(mesh, dims, mode="ground", length_frac=1 / 3, diam_frac=0.3, x_frac=0.34, yaw_deg=0.0,
-                     pitch_deg=0.0, ride_cells=0.0):
-    """Mesh -> grid matrix (column-major 16 floats) and scale (mesh.ts:354-379)."""
+                     pitch_deg=0.0, ride_cells=0.0, spanwise=False):
+    """Mesh -> grid matrix (column-major 16 floats) and scale (mesh.ts:354-379): scale to
+    length_frac nx ('ground') or to diam_frac ny over the mesh height ('center'), rotate about the
+    mid-height centre by yaw about y, then pitch about z (positive pitch lowers +x), translate to
+    x_frac nx, z = nz / 2 and, on the ground, put the lowest vertex at ride_cells (the lowest of every
+    k-th vertex, k = max(1, V // 200000), as the reference samples large meshes). spanwise stretches
+    z to span the tunnel (nz + 4 cells; the quasi-2D cylinder)."""
     nx, ny, nz = dims
     lo, hi = mesh.min, mesh.max
     height = hi[1] - lo[1]
@@ -338,11 +343,15 @@ def placement_matrix(mesh, dims, mode="ground", length_frac=1 / 3, diam_frac=0.3
     cy = (hi[1] + lo[1]) / 2
     R = np.eye(4)
     R[:3, :3] = rot_y(yaw_deg) @ rot_z(-pitch_deg)
-    S = np.diag([scale, scale, scale, 1.0])
+    sz = scale
+    if spanwise:
+        sz = (nz + 4) / max(hi[2] - lo[2], 1e-6) if nz > 1 else scale
+    S = np.diag([scale, scale, sz, 1.0])
     C = np.eye(4)
     C[1, 3] = -cy
     M = R @ S @ C
-    pts = np.c_[mesh.positions.astype(np.float64), np.ones(len(mesh.positions))] @ M.T
+    step = max(1, len(mesh.positions) // 200000)
+    pts = np.c_[mesh.positions[::step].astype(np.float64), np.ones(len(mesh.positions[::step]))] @ M.T
     min_y = pts[:, 1].min()
     zc = nz / 2 if nz > 1 else 0.5
     ty = -min_y + ride_cells if mode == "ground" else ny / 2 + 0.37
