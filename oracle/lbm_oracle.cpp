@@ -169,10 +169,10 @@ double bf16_margin(double v) {
   return std::fabs(t - std::floor(t) - 0.5) * spacing;
 }
 
-// The network on normalised inputs with the MFMA path's roundings: BF16 inputs, weights and
-// (after the bias, before the ReLU) first hidden layer; accumulation, second layer and output in
-// FP64 here. Returns nu_t = nu_scale * min(softplus(z), 10). margin (may be null): the smallest
-// distance of a BF16-rounded input or positive first-layer value to a rounding midpoint.
+// The network on normalised inputs with the MFMA path's roundings: BF16 inputs, weights (all three
+// layers) and hidden layers (after the bias, before the ReLU); accumulation and biases in FP64
+// here. Returns nu_t = nu_scale * min(softplus(z), 10). margin (may be null): the smallest distance
+// of a BF16-rounded input or positive hidden value to a rounding midpoint.
 double mlp_forward(const or_mlp& net, const double xn[8], double* margin = nullptr) {
   const double* p = net.params;
   double h1[32], h2[32];
@@ -185,14 +185,15 @@ double mlp_forward(const or_mlp& net, const double xn[8], double* margin = nullp
     if (margin && a > 0.0) mg = std::min(mg, bf16_margin(a));
     h1[o] = std::max(round_bf16(a), 0.0);
   }
-  if (margin) *margin = mg;
   for (int o = 0; o < 32; o++) {
     double a = p[1312 + o];
     for (int i = 0; i < 32; i++) a += round_bf16(p[288 + o * 32 + i]) * h1[i];
-    h2[o] = std::max(a, 0.0);
+    if (margin && a > 0.0) mg = std::min(mg, bf16_margin(a));
+    h2[o] = std::max(round_bf16(a), 0.0);
   }
+  if (margin) *margin = mg;
   double z = p[1376];
-  for (int i = 0; i < 32; i++) z += p[1344 + i] * h2[i];
+  for (int i = 0; i < 32; i++) z += round_bf16(p[1344 + i]) * h2[i];
   const double sp = z > 20.0 ? z : std::log1p(std::exp(z));
   return net.nu_scale * std::min(sp, 10.0);
 }
@@ -454,7 +455,7 @@ struct Solver {
       const double S6[6] = {sc * (double)Pi[0][0], sc * (double)Pi[1][1], sc * (double)Pi[2][2],
                             sc * (double)Pi[0][1], sc * (double)Pi[0][2], sc * (double)Pi[1][2]};
       double x[8], xn[8];
-      mlp_features(S6, wdist[idx] / 32.0, std::sqrt((double)usq), ((double)tau0 - 0.5) / 3.0, x);
+      mlp_features(S6, wdist[idx] / 32.0, std::sqrt((double)usq), prm.nu, x);
       for (int k = 0; k < 8; k++) xn[k] = (x[k] - net.mu[k]) / (net.sigma[k] > 0 ? net.sigma[k] : 1.0);
       const double nu = mlp_forward(net, xn, &margin[idx]);
       if (nu == nu && nu < 1e3) {

@@ -103,7 +103,7 @@ def forward_bf16(p, xn, nu_scale):
     W1, b1, W2, b2, w3, b3 = unpack(p)
     h1 = np.maximum(bf16(xn) @ bf16(W1).T + b1, 0)
     h2 = np.maximum(bf16(h1) @ bf16(W2).T + b2, 0)
-    z = h2 @ w3 + b3
+    z = bf16(h2) @ bf16(w3) + b3
     return nu_scale * np.minimum(softplus(z), TARGET_CLIP)
 
 

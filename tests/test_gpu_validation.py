@@ -75,12 +75,16 @@ def test_ahmed_against_fp64_oracle(prec):
 
 @pytest.mark.parametrize("prec", ["f32", "f16"])
 def test_ahmed_drag(prec):
-    # The reference's band (0.285 +- 35 %) is not reached at L = 64 cells and Re_sim = 2e4: the
-    # solver reads C_D near 0.49 (its own README notes that C_D reads high at these resolutions and
-    # publishes no Ahmed figure). What is asserted is a stable run, a converged window and the
-    # measured value, so that a change in the physics shows here.
+    """The Ahmed case at full size (valscale 1, L = 64 cells, Re_sim = 2e4) against the FP64 oracle's
+    run of it (about 52 minutes on 5 cores): the oracle reads C_D 0.490 +- 0.019, so the reference's
+    band (0.285 +- 35 %) is out of reach of its own method at this resolution too (its README notes
+    that C_D reads high and publishes no Ahmed figure). The GPU run must be stable, converged and
+    within 3 standard errors of the two windows plus 3 % of the oracle's value."""
+    ref = _oracle_fixture("ahmed_valscale_1")
     r = V.ahmed(_make(prec), scale=1.0)
     m = r["metrics"]
-    print(r)
+    tol = 3 * (m["se"] ** 2 + ref["se"] ** 2) ** 0.5 + 0.03 * ref["cd"]
+    print(f"ahmed L=64 {prec}: C_D {m['cd']:.4f} +- {m['se']:.4f} (oracle {ref['cd']:.4f} +- {ref['se']:.4f}), tol {tol:.4f}",
+          flush=True)
     assert np.isfinite(m["cd"]) and m["se"] < 0.05
-    assert 0.40 < m["cd"] < 0.60
+    assert abs(m["cd"] - ref["cd"]) < tol

@@ -546,16 +546,12 @@ __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict_
   // is one column of every product, so its inputs reach no other cell's output; a non-finite input
   // of a live cell gives a non-finite output, which keeps Smagorinsky.
   const int lane = threadIdx.x & 63;
-  const float inu0 = 3.0f / (P.tau0 - 0.5f);
   auto stage = [&](const Relax& m, uint32_t wdb, int slot) {
-    const float irho = 4.5f * fast_rcp(m.rho45);
-    const float sc = -1.5f * m.omega * irho;  // S = -3 Pi / (2 rho tau)
-    float x[MLP_IN];
-    mlp_raw_features(sc * m.pxx, sc * m.pyy, sc * m.pzz, sc * m.pxy, sc * m.pxz, sc * m.pyz, (float)wdb * (1.0f / 32.0f),
-                     __builtin_sqrtf(m.ux * m.ux + m.uy * m.uy + m.uz * m.uz), inu0, x);
+    // S = -3 Pi / (2 rho tau) = sc (-Pi), sc = 1.5 omega / rho = 6.75 omega / (4.5 rho)
+    const float sc = 6.75f * m.omega * fast_rcp(m.rho45);
     float v[MLP_IN];
-#pragma unroll
-    for (int i = 0; i < MLP_IN; i++) v[i] = (x[i] - net->norm.mu[i]) * net->norm.isig[i];
+    mlp_step_inputs(m.pxx, m.pyy, m.pzz, m.pxy, m.pxz, m.pyz, sc, wdb, fmaf(m.ux, m.ux, fmaf(m.uy, m.uy, m.uz * m.uz)), P.inu32,
+                    net->norm, v);
     uint32_t* dst = (uint32_t*)(feats + slot * MLP_IN);
 #pragma unroll
     for (int i = 0; i < MLP_IN / 2; i++) dst[i] = pack_bf16(v[2 * i], v[2 * i + 1]);
@@ -598,11 +594,10 @@ __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict_
     In in;
     if constexpr (!reg) bulk_load<L, S, CPT>(f2, kind, P, r2, in);
     else in = in0;  // unused: the regularized outputs follow from the moments
+    // t0 + 3 max(nu, floor nu_smag) = max(t0 + 3 nu, t0 + floor (tau_smag - t0)); the comparison is
+    // false for NaN as well
     auto closure_tau = [&](Relax& m, float nu) {
-      if (nu == nu && nu < 1e3f) {
-        const float nus_ = (fast_rcp(m.omega) - m.t0) * (1.0f / 3.0f);
-        retau(m, m.t0 + 3.0f * fmaxf(nu, floor_ * nus_), reg);
-      }
+      if (nu < 1e3f) retau(m, fmaxf(fmaf(3.0f, nu, m.t0), fmaf(floor_, fast_rcp(m.omega) - m.t0, m.t0)), reg);
     };
     closure_tau(ma, nu_a);
     if constexpr (CPT == 2) closure_tau(mb, nu_b);

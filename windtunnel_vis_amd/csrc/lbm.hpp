@@ -211,7 +211,7 @@ struct KParams {
   float invSponge;    // 1 / max(nx - spongeStart, 1)
   float invSpongeIn;  // 1 / max(spongeIn, 1)
   float smag;         // 18 sqrt(2) Cs^2
-  float pad2;
+  float inu32;        // 1 / (32 nu0): the learned closure's local-Reynolds input from the wall-distance byte
   FastDiv divx, divy;
 };
 

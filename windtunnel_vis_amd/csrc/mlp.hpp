@@ -29,6 +29,11 @@ constexpr float MLP_TARGET_CLIP = 10.0f;
 struct MlpNorm {
   float mu[MLP_IN];
   float isig[MLP_IN];  // 1 / sigma
+  // the step kernel's form of the normalisation, xn = fma(r, na, nb), on its raw inputs r: base-2
+  // logarithms for the compressed inputs (na = ln 2 / sigma), the wall-distance byte (na = 1 / (32
+  // sigma)); nb = -mu / sigma
+  float na[MLP_IN];
+  float nb[MLP_IN];
   float nu_scale;      // nu_t = nu_scale * min(softplus(z), MLP_TARGET_CLIP)
   float smag_floor;    // nu_t >= smag_floor * Smagorinsky's nu_t (the guard)
   float pad[2];
@@ -75,6 +80,33 @@ __host__ __device__ inline void mlp_raw_features(float sxx, float syy, float szz
   x[5] = slog(m, ir);
   x[6] = wd;
   x[7] = log1p_fast(umag * fmaxf(wd, 0.5f) * inu0);
+}
+
+// The step kernel's inputs, normalised, from the moments of a cell: the same eight inputs as
+// mlp_raw_features in a form with fewer instructions. P = -Pi^neq (pxx .. pyz negated: free source
+// modifiers), so S = sc P with sc = 3 / (2 rho tau) > 0; an invariant of degree k of S is sc^k times
+// that of P, so its compressed form is sign(I(P)) log2(1 + q_k |I(P)|) with q_k = (sc / S_REF)^k,
+// the natural logarithm's ln 2 folded into the normalisation (nm.na, nm.nb). wdb: wall distance byte
+// (32 per cell); usq: |u|^2; inu32: 1 / (32 nu0).
+__device__ __forceinline__ void mlp_step_inputs(float pxx, float pyy, float pzz, float pxy, float pxz, float pyz, float sc, uint32_t wdb,
+                                                float usq, float inu32, const MlpNorm& nm, float (&v)[MLP_IN]) {
+  const float a = -pxx, b = -pyy, c = -pzz, d = -pxy, e = -pxz, f = -pyz;
+  const float i1 = a + b + c;
+  const float tr2 = fmaf(a, a, fmaf(b, b, c * c)) + 2.0f * fmaf(d, d, fmaf(e, e, f * f));
+  const float i2 = 0.5f * fmaf(i1, i1, -tr2);
+  const float i3 = a * fmaf(b, c, -f * f) - d * fmaf(d, c, -f * e) + e * fmaf(d, f, -b * e);
+  const float tr3 = fmaf(i1 * i1, i1, 3.0f * fmaf(-i1, i2, i3));
+  const float q1 = sc * (1.0f / MLP_S_REF), q2 = q1 * q1, q3 = q2 * q1;
+  auto lg = [](float q, float x) { return __builtin_amdgcn_logf(fmaf(q, fabsf(x), 1.0f)); };  // argument >= 1
+  v[0] = fmaf(__builtin_copysignf(lg(q1, i1), i1), nm.na[0], nm.nb[0]);
+  v[1] = fmaf(__builtin_copysignf(lg(q2, i2), i2), nm.na[1], nm.nb[1]);
+  v[2] = fmaf(__builtin_copysignf(lg(q3, i3), i3), nm.na[2], nm.nb[2]);
+  v[3] = fmaf(lg(q2, tr2), nm.na[3], nm.nb[3]);
+  v[4] = fmaf(__builtin_copysignf(lg(q3, tr3), tr3), nm.na[4], nm.nb[4]);
+  v[5] = fmaf(lg(q1, __builtin_amdgcn_sqrtf(2.0f * tr2)), nm.na[5], nm.nb[5]);
+  const float wd = (float)wdb;
+  v[6] = fmaf(wd, nm.na[6], nm.nb[6]);
+  v[7] = fmaf(lg(__builtin_amdgcn_sqrtf(usq) * fmaxf(wd, 16.0f), inu32), nm.na[7], nm.nb[7]);
 }
 
 __host__ __device__ inline float softplus(float z) {
@@ -135,7 +167,7 @@ struct alignas(16) MlpDev {
   bf16x4_t w2a[4][64];  // A operands of layer 2 per K-chunk and lane
   float b1d[2][16];     // layer-1 bias in D layout, per lane half
   float b2d[2][16];     // layer-2 bias in D layout
-  float w3d[2][16];     // output weights in D layout
+  bf16x4_t w3a[4][2];   // A operand of the output layer per K-chunk and lane half (all 32 rows equal)
   float b3;
   float pad[3];
   MlpNorm norm;
@@ -176,11 +208,20 @@ __device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* 
     for (int r = 0; r < 16; r++) acc[r] = net->b2d[hg][r];
 #pragma unroll
     for (int c = 0; c < 4; c++) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2[c], hb[c], acc, 0, 0, 0);
-    float z = 0.0f;
+    // output layer on the matrix cores as well: z^T = w3 H2^T with w3 in every row of the A
+    // operand, so every register of every lane holds z of the lane's cell column
 #pragma unroll
-    for (int r = 0; r < 16; r++) z = fmaf(net->w3d[hg][r], relu(acc[r]), z);
-    z += __shfl_xor(z, 32);
-    if (h == 0) nu[cell] = net->norm.nu_scale * fminf(softplus(z + net->b3), MLP_TARGET_CLIP);
+    for (int c = 0; c < 4; c++) {
+      const uint32_t lo = relu_bf16x2(pack_bf16(acc[4 * c], acc[4 * c + 1]));
+      const uint32_t hi = relu_bf16x2(pack_bf16(acc[4 * c + 2], acc[4 * c + 3]));
+      hb[c] = __builtin_bit_cast(bf16x4_t, (uint64_t)lo | ((uint64_t)hi << 32));
+    }
+    f32x16_t out;
+#pragma unroll
+    for (int r = 0; r < 16; r++) out[r] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; c++) out = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(net->w3a[c][hg], hb[c], out, 0, 0, 0);
+    if (h == 0) nu[cell] = net->norm.nu_scale * fminf(softplus(out[0] + net->b3), MLP_TARGET_CLIP);
   }
 }
 

@@ -71,12 +71,19 @@ void build_dev(const wt_mlp* net, MlpDev* d) {
       const int i = 8 * (r / 4) + 4 * h + (r % 4);
       d->b1d[h][r] = p[MLP_OFF_B1 + i];
       d->b2d[h][r] = p[MLP_OFF_B2 + i];
-      d->w3d[h][r] = p[MLP_OFF_W3 + i];
     }
+  for (int c = 0; c < 4; c++)
+    for (int h = 0; h < 2; h++)
+      for (int t = 0; t < 4; t++) d->w3a[c][h][t] = host_bf16(p[MLP_OFF_W3 + 8 * c + 4 * h + t]);
   d->b3 = p[MLP_OFF_B3];
   for (int i = 0; i < MLP_IN; i++) {
+    const double isig = net->sigma[i] > 0.0f ? 1.0 / net->sigma[i] : 1.0;
     d->norm.mu[i] = net->mu[i];
-    d->norm.isig[i] = net->sigma[i] > 0.0f ? 1.0f / net->sigma[i] : 1.0f;
+    d->norm.isig[i] = (float)isig;
+    // the step kernel's raw inputs: base-2 logarithms, except the wall distance (its byte, 32 per cell)
+    const double unit = i == 6 ? 1.0 / 32.0 : 0.69314718055994530942;
+    d->norm.na[i] = (float)(unit * isig);
+    d->norm.nb[i] = (float)(-(double)net->mu[i] * isig);
   }
   d->norm.nu_scale = net->nu_scale;
   d->norm.smag_floor = net->smag_floor;

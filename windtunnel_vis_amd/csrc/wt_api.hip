@@ -52,6 +52,7 @@ void derive(wt_solver* s) {
   P.invSponge = (float)(1.0 / std::max((double)s->dims.nx - p.sponge_start * s->dims.nx, 1.0));
   P.invSpongeIn = (float)(1.0 / std::max(p.sponge_in * s->dims.nx, 1.0));
   P.smag = (float)(18.0 * std::sqrt(2.0) * p.cs * p.cs);
+  P.inu32 = (float)(1.0 / (32.0 * p.nu));
   P.divx.init((uint32_t)s->dims.nx);
   P.divy.init((uint32_t)s->dims.ny);
   P.cpt = (s->cpt == 2 && s->dims.nx % 2 == 0) ? 2 : 1;
