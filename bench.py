@@ -303,24 +303,28 @@ def closure_bench():
     reference C_D and the training data) and at 256x128x128 with plain Smagorinsky and with the
     trained network (windtunnel_vis_amd.closure_study)."""
     from windtunnel_vis_amd import closure_study
-    r = closure_study.study(fine_steps=24000, coarse_steps=24000, snapshot_from=10000, snapshot_every=2000, epochs=6)
+    r = closure_study.study()
+    rd = lambda v: None if v is None or v != v else round(v, 4)  # noqa: E731
 
-    def run(x):
-        return {"cd": round(x["cd_mean"], 4) if x["cd_mean"] == x["cd_mean"] else None, "converged": x["converged"],
-                "converged_s": None if x["converged_wall"] is None else round(x["converged_wall"], 3),
-                "within_2pct_of_fine_s": None if x["within2_wall"] is None else round(x["within2_wall"], 3),
-                "within_band_of_fine_s": {"%d%%" % round(100 * b): None if v is None else round(v, 3)
-                                          for b, v in x["band_wall"].items()},
+    def ens(x):
+        return {"cd": [rd(v) for v in x["cd"]], "mean": rd(x["mean"]), "se": rd(x["se"]), "err_vs_fine": rd(x["err"]),
+                "converged_at_end": x["converged"], "converged_s": [rd(v) for v in x["converged_wall"]],
+                "within_2pct_of_fine_s": [rd(v) for v in x["within2_wall"]], "within_2pct_runs": x["within2_runs"],
                 "mlups": round(x["mlups"], 1), "unstable": x["unstable"]}
 
-    return {"body": "truck + trailer", "fine": run(r["fine"]), "coarse_plain": run(r["coarse_plain"]),
-            "coarse_learned": run(r["coarse_learned"]), "cd_err_plain": round(r["cd_err_plain"], 4),
-            "cd_err_learned": round(r["cd_err_learned"], 4), "training_records": r["records"],
+    f = r["fine"]
+    return {"body": "truck + trailer", "fine": {"cd": rd(r["cd_fine"]), "cd_window": rd(f["cd_mean"]), "converged": f["converged"],
+                                                "converged_s": rd(f["converged_wall"]), "mlups": round(f["mlups"], 1)},
+            "coarse_plain": ens(r["plain"]), "coarse_learned": ens(r["learned"]),
+            "gain": rd(r["gain"]), "gain_se": rd(r["gain_se"]), "output_factor": rd(r["output_factor"]),
+            "calibration": [{"factor": c["scale"], "cd": [rd(v) for v in c["cd"]], "cd_mean": rd(c["mean"])} for c in r["calibration"]],
+            "learned_within_2pct_median_s": rd(r["learned_within2_median_s"]),
+            "speedup_vs_fine": rd(r.get("speedup_vs_fine")), "training_records": r["records"],
             "train_seconds": round(r["train_seconds"], 2), "train_loss": [round(x, 5) for x in r["train_loss"]],
             "smag_floor": r["network"].smag_floor, "nu_scale": r["network"].nu_scale,
-            "ensemble_cd": {k: [round(x, 4) for x in v] for k, v in r["ensemble"].items()},
-            "ensemble_note": "coarse runs from seeded perturbed starts (1 % of U): their spread is the scatter of one "
-                             "converged-window C_D"}
+            "note": "coarse runs of %d steps from seeded perturbed starts (1 %% of U), C_D over each run's second half; "
+                    "calibration seeds %s, evaluation seeds %s; gain = |plain mean - fine| - |learned mean - fine|"
+                    % (r["coarse_steps"], r["calib_seeds"], r["eval_seeds"])}
 
 
 def cpu_threads():

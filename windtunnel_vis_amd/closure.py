@@ -95,18 +95,66 @@ class Dataset:
             pass
 
 
-def fit_normalization(net, dataset, samples=200000, seed=1):
-    """Input mean / std and the output scale from (a sample of) the dataset."""
+def _sample(dataset, samples, seed):
     n = dataset.count
-    recs = dataset.read(0, n) if n <= samples else dataset.read(0, n)[np.random.default_rng(seed).choice(n, samples, replace=False)]
+    if n <= samples:
+        return dataset.read(0, n)
+    idx = np.sort(np.random.default_rng(seed).choice(n, samples, replace=False))
+    # read in a few large slices rather than the whole set at once
+    out = np.empty((samples, 10), np.float32)
+    step = 1 << 22
+    k = 0
+    for a in range(0, n, step):
+        sel = idx[(idx >= a) & (idx < a + step)]
+        if len(sel):
+            blk = dataset.read(a, min(step, n - a))
+            out[k:k + len(sel)] = blk[sel - a]
+            k += len(sel)
+    return out
+
+
+def fit_normalization(net, dataset, samples=1_000_000, seed=1, quantile=99.9, headroom=8.0):
+    """Input mean / std and the output scale from a sample of the dataset. The output is capped at
+    MLP_TARGET_CLIP = 10 nu_scale, so nu_scale is set to the `quantile` of the targets over
+    `headroom`: the cap lies above the tail of the targets, where the eddy viscosity matters (wake,
+    beside the body), while the bulk of the (uniform-flow) targets near zero still resolves."""
+    recs = _sample(dataset, samples, seed)
     x = recs[:, :8].astype(np.float64)
     net.mu = x.mean(0).astype(np.float32)
     sd = x.std(0)
     net.sigma = np.where(sd > 1e-12, sd, 1.0).astype(np.float32)
     t = recs[:, 8].astype(np.float64)
-    scale = float(np.percentile(t[t > 0], 90)) if np.any(t > 0) else 1e-3
+    scale = float(np.percentile(t, quantile)) / headroom if np.any(t > 0) else 1e-3
     net.nu_scale = max(scale, 1e-7)
     return net
+
+
+def wall_weight(d):
+    """Weight of the near-wall term: 1 within two cells of the body, falling linearly to 0 at four
+    (the wall-distance input saturates at four cells)."""
+    return np.clip((4.0 - np.asarray(d, np.float64)) / 2.0, 0.0, 1.0)
+
+
+def smagorinsky_nu(raw, cs):
+    """The coarse grid's Smagorinsky eddy viscosity (Cs Delta)^2 |S| (Delta = one cell) from records'
+    |S| input (x5 = log(1 + |S| / 1e-3))."""
+    return cs * cs * 1e-3 * np.expm1(np.abs(np.asarray(raw, np.float64)[:, 5]))
+
+
+def near_wall_dataset(dataset, cs, chunk=1 << 22):
+    """A copy of `dataset` whose targets carry, besides the filtered subgrid stress, the coarse
+    Smagorinsky eddy viscosity weighted toward the body: t + wall_weight(d) (Cs Delta)^2 |S|. The
+    2x-filtered fine field holds little subgrid stress (its scales are resolved on the coarse grid),
+    so the error of the coarse run sits at the body, where the coarse grid under-resolves the
+    boundary layer and the separation; this term gives the network a near-wall eddy viscosity whose
+    size the output calibration sets."""
+    n = dataset.count
+    out = Dataset(max(n, 1))
+    for a in range(0, n, chunk):
+        r = dataset.read(a, min(chunk, n - a))
+        r[:, 8] = (r[:, 8].astype(np.float64) + wall_weight(r[:, 6]) * smagorinsky_nu(r, cs)).astype(np.float32)
+        out.append(r)
+    return out
 
 
 def train(dataset, net, epochs=10, batch=4096, lr=2e-3, seed=20261203, max_samples=0, beta1=0.9, beta2=0.999, eps=1e-8):
