@@ -211,30 +211,38 @@ def copy_probe(nbytes=2 << 30, iters=5):
     return g.value
 
 
-def physics(s, body, info, u, total=5000, chunk=250):
-    """C_D / C_L averaged over the second half of a `total`-step run (harness.ts:172-183)."""
-    from windtunnel_vis_amd.analysis import coefficients
+def physics(s, body, info, u, max_steps=40000, chunk=250):
+    """C_D / C_L of a run from the init state, read as the app reads them (app.ts:447-468): the
+    window of the last 4 L/U, stopped when the convergence rule holds there (span >= 3 L/U, SE <
+    max(2 % |mean|, 0.005)) or after max_steps."""
+    from windtunnel_vis_amd.analysis import coefficients, converged, windowStats
     s.reset()
-    acc_d, acc_l, k = 0.0, 0.0, 0
-    unstable = False
-    for st_ in range(0, total, chunk):
+    t, cd, cl = [], [], []
+    ok, unstable, step = False, False, 0
+    L = body.length
+    while step < max_steps and not ok:
         r = s.stepAndSample(chunk)
-        c = coefficients(r.fx, r.fy, r.fz, u, info["frontal"])
-        if st_ > total / 2:
-            acc_d += c["cd"]
-            acc_l += c["cl"]
-            k += 1
+        step += chunk
         if r.unstable:
             unstable = True
             break
-    return dict(cd=acc_d / max(k, 1), cl=acc_l / max(k, 1), unstable=unstable, frontal=info["frontal"])
+        c = coefficients(r.fx, r.fy, r.fz, u, info["frontal"])
+        t.append(step)
+        cd.append(c["cd"])
+        cl.append(c["cl"])
+        ok, w = converged(t, cd, L, u)
+    ok, w = converged(t, cd, L, u)
+    wl = windowStats(t, cl, 4 * L / u)
+    return dict(cd=w["mean"], cd_se=w["se"], cl=wl["mean"], converged=ok, steps=step, unstable=unstable,
+                frontal=info["frontal"], L=L)
 
 
-def sweep_bench(steps=500, total=2000, chunk=250):
+def sweep_bench(steps=500, max_steps=16000, chunk=250):
     """Batched yaw / ride-height sweep: 24 uneven D3Q19 FP16 domains (autotuned launch configuration
     per domain) stepped by one launch per step in 100-step hipGraph chunks. Aggregate MLUPS (batched
     with graphs, batched direct launches, and the same domains stepped one after another), then
-    per-case C_D / C_L over the second half of `total` steps."""
+    per-case C_D / C_L read as the app reads them (window of the last 4 L/U), the batch stepped until
+    the convergence rule holds in every case or `max_steps`."""
     from windtunnel_vis_amd import BatchGPU, sweep, tuning
     from windtunnel_vis_amd.analysis import coefficients
     cases = sweep.sweep_cases()
@@ -273,29 +281,41 @@ def sweep_bench(steps=500, total=2000, chunk=250):
     for s_ in solvers:
         s_.reset()
     u = sweep.sweep_params()["u"]
-    acc = [[0.0, 0.0, 0] for _ in solvers]
+    from windtunnel_vis_amd.analysis import converged, windowStats
+    series = [([], [], []) for _ in solvers]
     unstable = [False] * len(solvers)
-    for k in range(0, total, chunk):
+    done = [False] * len(solvers)
+    conv_step = [None] * len(solvers)
+    step = 0
+    while step < max_steps and not all(done[i] or unstable[i] for i in range(len(solvers))):
         batch.run(chunk)
-        st = batch.stats()
-        if k + chunk > total // 2:
-            for i, (r, (s_, body, info)) in enumerate(zip(st, made)):
-                c = coefficients(r.fx, r.fy, r.fz, u, info["frontal"])
-                acc[i][0] += c["cd"]
-                acc[i][1] += c["cl"]
-                acc[i][2] += 1
-                unstable[i] |= r.unstable
+        step += chunk
+        for i, (r, (s_, body, info)) in enumerate(zip(batch.stats(), made)):
+            unstable[i] |= r.unstable
+            c = coefficients(r.fx, r.fy, r.fz, u, info["frontal"])
+            t_, d_, l_ = series[i]
+            t_.append(step)
+            d_.append(c["cd"])
+            l_.append(c["cl"])
+            ok, _ = converged(t_, d_, body.length, u)
+            if ok and not done[i]:
+                done[i], conv_step[i] = True, step
     batch.destroy()
-    per_case = [{"dims": list(c["dims"]), "body": c["kind"], "yaw": c["yaw"], "ride": c["ride"],
-                 "cd": round(a[0] / max(a[2], 1), 4), "cl": round(a[1] / max(a[2], 1), 4), "unstable": u_}
-                for c, a, u_ in zip(cases, acc, unstable)]
+    per_case = []
+    for c, (s_, body, info), (t_, d_, l_), u_, cs_ in zip(cases, made, series, unstable, conv_step):
+        ok, w = converged(t_, d_, body.length, u)
+        wl = windowStats(t_, l_, 4 * body.length / u)
+        rd = lambda v: round(v, 4) if v == v else None  # noqa: E731
+        per_case.append({"dims": list(c["dims"]), "body": c["kind"], "yaw": c["yaw"], "ride": c["ride"],
+                         "cd": rd(w["mean"]), "cd_se": rd(w["se"]), "cl": rd(wl["mean"]), "converged_step": cs_,
+                         "converged_at_end": ok, "unstable": u_})
     for s_ in solvers:
         s_.destroy()
     return {"domains": len(cases), "cells": cells, "steps": steps,
             "mlups_batched_graph": round(cells * steps / t_graph / 1e6, 1),
             "mlups_batched_direct": round(cells * steps / t_direct / 1e6, 1),
             "mlups_serial_per_domain": round(cells * steps / t_serial / 1e6, 1),
-            "cd_steps": total, "cases": per_case}
+            "cd_steps": step, "cases_converged": sum(c["converged_step"] is not None for c in per_case), "cases": per_case}
 
 
 def closure_bench():
@@ -442,8 +462,10 @@ def main():
                 "note": "the 318.8 MB ping-pong partly fits the 256 MB memory-side cache: not a roofline figure"}
         if not args.no_physics:
             ph = physics(rs["solver"], rs["body"], rs["info"], 0.08)
-            line["physics"] = {"cd": round(ph["cd"], 4), "cl": round(ph["cl"], 4), "frontal_cells": ph["frontal"],
-                               "unstable": ph["unstable"], "steps": 5000}
+            rd = lambda v: round(v, 4) if v == v else None  # noqa: E731
+            line["physics"] = {"cd": rd(ph["cd"]), "cd_se": rd(ph["cd_se"]), "cl": rd(ph["cl"]),
+                               "converged": ph["converged"], "steps": ph["steps"], "frontal_cells": ph["frontal"],
+                               "unstable": ph["unstable"], "window": "last 4 L/U (L = %g cells)" % ph["L"]}
         out["d3q19_%s_256x128x128" % args.precision] = line
         plain_ms = rs["k_avg_ms"]
         rs["solver"].destroy()

@@ -152,6 +152,63 @@ def test_collision_stress_relaxation_and_smagorinsky(lat, dims, cs):
             assert np.allclose(f1 - fe, (1 - 1 / tau) * (f0[:, inner] - feq(lat, r0, u0)), atol=1e-14)
 
 
+@pytest.mark.parametrize("sides", ["periodic", "freeslip"])
+def test_free_slip_edge_mirrors(sides):
+    """One BGK step of a spatially uniform but otherwise generic state (every direction its own value)
+    in a box periodic in x, with free-slip y walls and periodic or free-slip z sides
+    (wgslLbm.ts:153-170): a pull from outside in y alone takes the y-mirrored direction at the same
+    x and clamped z; from outside in y and z at once (the y-z edges) the direction mirrored in both y
+    and z at the cell itself, with periodic sides too; from outside in z alone the periodic image or
+    the z-mirrored direction. Every pulled value is one of the uniform state's, so the expected
+    populations follow from the mirror tables built from the velocity set; a mirror-y pull on the
+    y-z edges, or a periodic pull there, fails the test."""
+    lat = D3Q19
+    dims = (5, 4, 3)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    c = lat.c.astype(int)
+    rng = np.random.default_rng(7)
+    g = lat.w * (1.0 + 0.2 * rng.uniform(-1, 1, lat.q))
+    f0 = np.tile(g[:, None], (1, n))
+
+    def find(v):
+        return next(j for j in range(lat.q) if (c[j] == v).all())
+
+    mir_y = [find(c[i] * [1, -1, 1]) for i in range(lat.q)]
+    mir_z = [find(c[i] * [1, 1, -1]) for i in range(lat.q)]
+    mir_yz = [find(c[i] * [1, -1, -1]) for i in range(lat.q)]
+    tau = 0.8
+    o = OracleSolver(dims, dict(u=0.05, nu=(tau - 0.5) / 3, cs=0.0, ground="freeslip", sides=sides, spongeNu=0.0,
+                                spongeIn=0.0, collision="bgk"), periodic_x=True)
+    o.uploadPopulations(f0)
+    o.step(1)
+    f1 = o.readPopulations()
+    expect = np.empty_like(f1)
+    for z in range(nz):
+        for y in range(ny):
+            pulled = np.empty(lat.q)
+            for i in range(lat.q):
+                sy, sz = y - c[i, 1], z - c[i, 2]
+                y_out, z_out = not 0 <= sy < ny, not 0 <= sz < nz
+                if y_out and z_out:
+                    j = mir_yz[i]
+                elif y_out:
+                    j = mir_y[i]
+                elif z_out:
+                    j = i if sides == "periodic" else mir_z[i]
+                else:
+                    j = i
+                pulled[i] = g[j]
+            rho = pulled.sum()
+            u = (c.T.astype(np.float64) @ pulled) / rho
+            post = pulled + (feq(lat, np.array([rho]), u[:, None])[:, 0] - pulled) / tau
+            for x in range(nx):
+                expect[:, x + nx * (y + ny * z)] = post
+    assert np.abs(f1 - expect).max() < 1e-15
+    # the y-z edges are where the rule matters: there the mirror-y pull would differ
+    assert any(g[mir_yz[i]] != g[mir_y[i]] for i in range(lat.q) if c[i, 1] != 0 and c[i, 2] != 0)
+
+
 @pytest.mark.parametrize("coll", ["bgk", "regularized"])
 def test_shear_wave_decay_gives_viscosity(coll):
     """u_x(z) = A sin(2 pi z / L) in a box periodic in x and z decays as exp(-nu k^2 t),
