@@ -119,19 +119,27 @@ __host__ __device__ inline float softplus(float z) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Wave-wide BF16 MFMA inference (v_mfma_f32_32x32x8bf16_1k). A wave evaluates 128 cells in four
-// groups of 32, in transposed form so that no activation is moved between lanes:
-//   H1^T (32 x 32 cells) = W1 (32 x 8) X^T (8 x 32 cells) + b1,   relu
-//   H2^T (32 x 32 cells) = W2 (32 x 32) H1^T + b2 (four K = 8 chunks),   relu
-//   z = w3 . h2 + b3 (16 hidden units per lane, the other 16 in lane l ^ 32),
-//   nu_t = s min(softplus(z), MLP_TARGET_CLIP).
-// Operand layouts (checked on gfx950): A (32 x 8): lane l holds row l % 32, k = 4 (l / 32) + 0..3;
-// B (8 x 32): lane l holds column l % 32, the same k; D (32 x 32): lane l, register r holds row
-// 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32. Register block c of D (r = 4c..4c+3) is therefore
-// exactly the B operand of K-chunk c of the next layer.
-// ---------------------------------------------------------------------------------------------
+// Wave-wide BF16 MFMA inference. A wave evaluates 128 cells in four groups of 32, in transposed
+// form so that no activation is moved between lanes:
+//   H1^T (32 x 32 cells) = W1 (32 x 8) X^T (8 x 32 cells) + b1,  relu   (v_mfma_f32_32x32x8bf16_1k)
+//   H2^T (32 x 32 cells) = W2 (32 x 32) H1^T + b2, relu       (two K = 16 chunks, v_mfma_f32_32x32x16_bf16)
+//   z^T (32 x 32 cells, all rows equal) = w3 (every row) H2^T  (two K = 16 chunks)
+//   nu_t = s min(softplus(z + b3), MLP_TARGET_CLIP).
+// Operand layouts (checked on gfx950): A (32 x K): lane l holds row l % 32 and K / 2 consecutive
+// k-slots from (K / 2)(l / 32); B (K x 32): lane l holds column l % 32 and the same k-slots; D
+// (32 x 32): lane l, register r holds row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32. Register
+// block c of D (r = 4c..4c+3) is therefore the B operand of K-chunk c of a K = 8 product, and
+// registers 8c..8c+7 the B operand of K-chunk c of a K = 16 product whose k-slot j of lane half h
+// stands for hidden unit 16c + 8 (j / 4) + 4h + j % 4 (mlp_hidden_of): the reduction order is
+// free, so the A operands are laid out in that order on the host.
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef short bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_hw __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4_hw __attribute__((ext_vector_type(4)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// hidden unit carried by k-slot j of lane half h in K = 16 chunk c (see the layouts above)
+__host__ __device__ constexpr int mlp_hidden_of(int c, int h, int j) { return 16 * c + 8 * (j / 4) + 4 * h + j % 4; }
 
 __device__ inline short to_bf16(float x) {  // round to nearest even (finite inputs)
   uint32_t u = __float_as_uint(x);
@@ -164,10 +172,10 @@ __device__ inline uint32_t relu_bf16x2(uint32_t u) {
 // Device-side network in the layout the MFMA path reads (prepared once from the FP32 master).
 struct alignas(16) MlpDev {
   bf16x4_t w1a[64];     // A operand of layer 1 per lane
-  bf16x4_t w2a[4][64];  // A operands of layer 2 per K-chunk and lane
+  bf16x8_t w2b[2][64];  // A operands of layer 2 per K = 16 chunk and lane
   float b1d[2][16];     // layer-1 bias in D layout, per lane half
   float b2d[2][16];     // layer-2 bias in D layout
-  bf16x4_t w3a[4][2];   // A operand of the output layer per K-chunk and lane half (all 32 rows equal)
+  bf16x8_t w3b[2][2];   // A operand of the output layer per K = 16 chunk and lane half (all 32 rows equal)
   float b3;
   float pad[3];
   MlpNorm norm;
@@ -177,13 +185,18 @@ struct alignas(16) MlpDev {
 // stage_net); feats: this wave's LDS staging area of 128 x 8 BF16 (cell-major; every lane wrote
 // its own cells before the call); nu: 128 floats of LDS for the results. All 64 lanes must call
 // it (MFMA uses the whole wave); cells are indexed 0..127.
+__device__ inline bf16x8_hw relu_pack8(const f32x16_t& acc, int c) {  // D registers 8c..8c+7 -> K = 16 B operand
+  u32x4_hw w;
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] = relu_bf16x2(pack_bf16(acc[8 * c + 2 * i], acc[8 * c + 2 * i + 1]));
+  return __builtin_bit_cast(bf16x8_hw, w);
+}
+
 __device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* feats, float* nu) {
   const int l = threadIdx.x & 63;
   const int h = l >> 5;
   const bf16x4_t w1 = net->w1a[l];
-  bf16x4_t w2[4];
-#pragma unroll
-  for (int c = 0; c < 4; c++) w2[c] = net->w2a[c][l];
+  const bf16x8_hw w2[2] = {__builtin_bit_cast(bf16x8_hw, net->w2b[0][l]), __builtin_bit_cast(bf16x8_hw, net->w2b[1][l])};
   // one group of 32 cells at a time, the biases and output weights read from LDS in each pass:
   // kept in registers across the groups (48 of them, plus interleaved accumulators) they would
   // set the register budget of a kernel that fuses this evaluation
@@ -197,30 +210,18 @@ __device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* 
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = net->b1d[hg][r];
     acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1, xb, acc, 0, 0, 0);
-    bf16x4_t hb[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const uint32_t lo = relu_bf16x2(pack_bf16(acc[4 * c], acc[4 * c + 1]));
-      const uint32_t hi = relu_bf16x2(pack_bf16(acc[4 * c + 2], acc[4 * c + 3]));
-      hb[c] = __builtin_bit_cast(bf16x4_t, (uint64_t)lo | ((uint64_t)hi << 32));
-    }
+    bf16x8_hw hb0 = relu_pack8(acc, 0), hb1 = relu_pack8(acc, 1);
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = net->b2d[hg][r];
-#pragma unroll
-    for (int c = 0; c < 4; c++) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2[c], hb[c], acc, 0, 0, 0);
-    // output layer on the matrix cores as well: z^T = w3 H2^T with w3 in every row of the A
-    // operand, so every register of every lane holds z of the lane's cell column
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const uint32_t lo = relu_bf16x2(pack_bf16(acc[4 * c], acc[4 * c + 1]));
-      const uint32_t hi = relu_bf16x2(pack_bf16(acc[4 * c + 2], acc[4 * c + 3]));
-      hb[c] = __builtin_bit_cast(bf16x4_t, (uint64_t)lo | ((uint64_t)hi << 32));
-    }
-    f32x16_t out;
-#pragma unroll
-    for (int r = 0; r < 16; r++) out[r] = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 4; c++) out = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(net->w3a[c][hg], hb[c], out, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2[0], hb0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2[1], hb1, acc, 0, 0, 0);
+    // output layer on the matrix cores as well: w3 in every row of the A operand, so every register
+    // of every lane holds z of the lane's cell column
+    hb0 = relu_pack8(acc, 0);
+    hb1 = relu_pack8(acc, 1);
+    f32x16_t out = {};
+    out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w3b[0][hg]), hb0, out, 0, 0, 0);
+    out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w3b[1][hg]), hb1, out, 0, 0, 0);
     if (h == 0) nu[cell] = net->norm.nu_scale * fminf(softplus(out[0] + net->b3), MLP_TARGET_CLIP);
   }
 }

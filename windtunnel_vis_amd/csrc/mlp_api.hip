@@ -61,10 +61,9 @@ void build_dev(const wt_mlp* net, MlpDev* d) {
   const float* p = net->params;
   for (int l = 0; l < 64; l++) {
     const int row = l % 32, h = l / 32;
-    for (int t = 0; t < 4; t++) {
-      d->w1a[l][t] = host_bf16(p[MLP_OFF_W1 + row * MLP_IN + 4 * h + t]);
-      for (int c = 0; c < 4; c++) d->w2a[c][l][t] = host_bf16(p[MLP_OFF_W2 + row * MLP_H + 8 * c + 4 * h + t]);
-    }
+    for (int t = 0; t < 4; t++) d->w1a[l][t] = host_bf16(p[MLP_OFF_W1 + row * MLP_IN + 4 * h + t]);
+    for (int c = 0; c < 2; c++)
+      for (int j = 0; j < 8; j++) d->w2b[c][l][j] = host_bf16(p[MLP_OFF_W2 + row * MLP_H + mlp_hidden_of(c, h, j)]);
   }
   for (int h = 0; h < 2; h++)
     for (int r = 0; r < 16; r++) {
@@ -72,9 +71,9 @@ void build_dev(const wt_mlp* net, MlpDev* d) {
       d->b1d[h][r] = p[MLP_OFF_B1 + i];
       d->b2d[h][r] = p[MLP_OFF_B2 + i];
     }
-  for (int c = 0; c < 4; c++)
+  for (int c = 0; c < 2; c++)
     for (int h = 0; h < 2; h++)
-      for (int t = 0; t < 4; t++) d->w3a[c][h][t] = host_bf16(p[MLP_OFF_W3 + 8 * c + 4 * h + t]);
+      for (int j = 0; j < 8; j++) d->w3b[c][h][j] = host_bf16(p[MLP_OFF_W3 + mlp_hidden_of(c, h, j)]);
   d->b3 = p[MLP_OFF_B3];
   for (int i = 0; i < MLP_IN; i++) {
     const double isig = net->sigma[i] > 0.0f ? 1.0 / net->sigma[i] : 1.0;
