@@ -212,9 +212,10 @@ def copy_probe(nbytes=2 << 30, iters=5):
 
 
 def physics(s, body, info, u, max_steps=40000, chunk=250):
-    """C_D / C_L of a run from the init state, read as the app reads them (app.ts:447-468): the
-    window of the last 4 L/U, stopped when the convergence rule holds there (span >= 3 L/U, SE <
-    max(2 % |mean|, 0.005)) or after max_steps."""
+    """C_D / C_L of a run from the init state, read as the app reads them (app.ts:447-468): samples
+    from one convective time on (the app's averaging start after a reset), the window of the last
+    4 L/U, stopped when the convergence rule holds there (span >= 3 L/U, SE < max(2 % |mean|,
+    0.005)) or after max_steps."""
     from windtunnel_vis_amd.analysis import coefficients, converged, windowStats
     s.reset()
     t, cd, cl = [], [], []
@@ -226,6 +227,8 @@ def physics(s, body, info, u, max_steps=40000, chunk=250):
         if r.unstable:
             unstable = True
             break
+        if step < L / u:
+            continue
         c = coefficients(r.fx, r.fy, r.fz, u, info["frontal"])
         t.append(step)
         cd.append(c["cd"])
@@ -241,8 +244,8 @@ def sweep_bench(steps=500, max_steps=16000, chunk=250):
     """Batched yaw / ride-height sweep: 24 uneven D3Q19 FP16 domains (autotuned launch configuration
     per domain) stepped by one launch per step in 100-step hipGraph chunks. Aggregate MLUPS (batched
     with graphs, batched direct launches, and the same domains stepped one after another), then
-    per-case C_D / C_L read as the app reads them (window of the last 4 L/U), the batch stepped until
-    the convergence rule holds in every case or `max_steps`."""
+    per-case C_D / C_L read as the app reads them (samples from one convective time on, window of the
+    last 4 L/U), the batch stepped until the convergence rule holds in every case or `max_steps`."""
     from windtunnel_vis_amd import BatchGPU, sweep, tuning
     from windtunnel_vis_amd.analysis import coefficients
     cases = sweep.sweep_cases()
@@ -292,6 +295,8 @@ def sweep_bench(steps=500, max_steps=16000, chunk=250):
         step += chunk
         for i, (r, (s_, body, info)) in enumerate(zip(batch.stats(), made)):
             unstable[i] |= r.unstable
+            if step < body.length / u:
+                continue
             c = coefficients(r.fx, r.fy, r.fz, u, info["frontal"])
             t_, d_, l_ = series[i]
             t_.append(step)
