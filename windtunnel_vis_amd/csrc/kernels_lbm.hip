@@ -694,15 +694,29 @@ __device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin
   }
 }
 
+// What block b of a step launch does: launch blocks come in groups of 8 (one per XCD), each group
+// 8 wall blocks or 8 bulk blocks in the order of the role table, which puts every wall group just
+// after the bulk group that streams the rows its links gather from (see build_roles): the wall
+// cells' scattered reads and partial-line writes then meet lines the bulk blocks moved through the
+// caches moments before instead of going to HBM on their own. The index keeps b's XCD (b % 8).
+struct BlockRole {
+  bool wall;
+  uint32_t index;  // wall block or bulk block index
+};
+__device__ __forceinline__ BlockRole block_role(const StepArgs& A, uint32_t b) {
+  const uint32_t r = __builtin_amdgcn_readfirstlane(glob(A.role)[b >> 3]);
+  return {(r >> 31) != 0u, (r & 0x7fffffffu) * 8u + (b & 7u)};
+}
+
 template <class L, class S, int CPT, bool REG>
 __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                                                 const uint8_t* __restrict__ kind, KParams P, StepArgs A) {
-  const uint32_t b = blockIdx.x;
-  if (b < A.nwb) {
-    wall_cell<L, S>(fin, fout, A, P, (int64_t)b * BLOCK + threadIdx.x);
+  const BlockRole br = block_role(A, blockIdx.x);
+  if (br.wall) {
+    wall_cell<L, S>(fin, fout, A, P, (int64_t)br.index * BLOCK + threadIdx.x);
     return;
   }
-  bulk_block<L, S, CPT, REG>(fin, fout, kind, P, A, b - A.nwb);
+  bulk_block<L, S, CPT, REG>(fin, fout, kind, P, A, br.index);
 }
 
 // The step with the learned closure in the bulk (D3Q19); wall cells keep Smagorinsky.
@@ -713,22 +727,23 @@ __global__ __launch_bounds__(BLOCK, 5) void k_step_mlp(const typename S::T* __re
   __shared__ short feats[BLOCK / 64][128 * MLP_IN];
   __shared__ float nus[BLOCK / 64][128];
   __shared__ MlpDev snet;
-  const uint32_t b = blockIdx.x;
-  if (b < A.nwb) {
-    wall_cell<D3Q19, S>(fin, fout, A, P, (int64_t)b * BLOCK + threadIdx.x);
+  const BlockRole br = block_role(A, blockIdx.x);
+  if (br.wall) {
+    wall_cell<D3Q19, S>(fin, fout, A, P, (int64_t)br.index * BLOCK + threadIdx.x);
     return;
   }
   stage_net(net, &snet);  // bulk blocks only: the whole block passes here
   const int w = threadIdx.x >> 6;
-  bulk_block<D3Q19, S, CPT, REG, true>(fin, fout, kind, P, A, b - A.nwb, wdist, &snet, feats[w], nus[w]);
+  bulk_block<D3Q19, S, CPT, REG, true>(fin, fout, kind, P, A, br.index, wdist, &snet, feats[w], nus[w]);
 }
 
 // One lattice step of many independent domains in one launch (sweeps): the same wall and bulk
 // work as k_step, each block finding its domain through `dmap` (block -> domain) and its
-// parameters, layout and buffers in the descriptor table. All domains' wall blocks come first.
+// parameters, layout, buffers and role table in the descriptor table. Each domain's blocks form
+// one contiguous run starting at a multiple of 8.
 template <class L, class S, int CPT, bool REG>
 __global__ __launch_bounds__(BLOCK, batch_waves<S>()) void k_step_batch(const DomainDesc* __restrict__ D, const uint32_t* __restrict__ dmap,
-                                                                      uint32_t wall_total, int32_t parity) {
+                                                                      int32_t parity) {
   using T = typename S::T;
   const uint32_t b = blockIdx.x;
   const uint32_t d = __builtin_amdgcn_readfirstlane(dmap[b]);
@@ -736,11 +751,12 @@ __global__ __launch_bounds__(BLOCK, batch_waves<S>()) void k_step_batch(const Do
   // (the pointers read from the table are accessed through glob(), see lbm.hpp)
   const T* fin = (const T*)dd.f[parity];
   T* fout = (T*)dd.f[parity ^ 1];
-  if (b < wall_total) {
-    wall_cell<L, S>(fin, fout, dd.A, dd.P, (int64_t)(b - dd.wall_first) * BLOCK + threadIdx.x);
+  const BlockRole br = block_role(dd.A, b - dd.first);
+  if (br.wall) {
+    wall_cell<L, S>(fin, fout, dd.A, dd.P, (int64_t)br.index * BLOCK + threadIdx.x);
     return;
   }
-  bulk_block<L, S, CPT, REG>(fin, fout, dd.kind, dd.P, dd.A, b - dd.bulk_first);
+  bulk_block<L, S, CPT, REG>(fin, fout, dd.kind, dd.P, dd.A, br.index);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1053,6 +1069,29 @@ void pull_table(const KParams& P, int64_t* out) {
   else go(D3Q19{});
 }
 
+static_assert(WALL_GROUP_CELLS == 8 * BLOCK, "a wall group is 8 wall blocks");
+
+std::vector<uint32_t> build_roles(const StepArgs& A, const std::vector<uint32_t>& wall_rows, int ny, int nz) {
+  const uint32_t nbg = A.nbb / 8, nwg = A.nwb / 8;
+  const int64_t rows = (int64_t)ny * nz;
+  // bulk group (launch order, per XCD) that holds the last item of each wall group's target row
+  std::vector<std::vector<uint32_t>> after(nbg + 1);
+  for (uint32_t k = 0; k < nwg; k++) {
+    const int64_t row = k < wall_rows.size() ? std::min<int64_t>((int64_t)wall_rows[k] + ny + 1, rows - 1) : rows - 1;
+    const int64_t item = std::min<int64_t>(row * A.wpr + A.wpr - 1, (int64_t)A.nwaves - 1);
+    const int64_t run = item / A.xk;  // runs of xk items, dealt round-robin to the XCDs (bulk_block)
+    const int64_t lw = (run / 8) * A.xk + item % A.xk;  // wave index on its XCD
+    after[std::min<int64_t>(lw / 4, nbg)].push_back(k);  // bulk group = (bulk block) / 8 = lw / 4
+  }
+  std::vector<uint32_t> role;
+  role.reserve(nbg + nwg);
+  for (uint32_t g = 0; g <= nbg; g++) {
+    if (g < nbg) role.push_back(g);
+    for (uint32_t k : after[g]) role.push_back(0x80000000u | k);
+  }
+  return role;
+}
+
 StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
                    int64_t chunk) {
   StepArgs A{};
@@ -1079,10 +1118,10 @@ StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* met
 }
 
 void launch_step_batch(hipStream_t st, int q, int prec, int cpt, bool reg, const DomainDesc* descs, const uint32_t* dmap,
-                       uint32_t wall_total, uint32_t blocks, int parity) {
+                       uint32_t blocks, int parity) {
   dispatch(q, prec, [&](auto l, auto s) {
     using L = decltype(l); using S = decltype(s);
-    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), 0, st, descs, dmap, wall_total, parity); };
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), 0, st, descs, dmap, parity); };
     // 16-bit words hold two cells: those formats always run two cells per lane
     if (S::BYTES == 2 || cpt == 2) {
       if (reg) go(k_step_batch<L, S, 2, true>);

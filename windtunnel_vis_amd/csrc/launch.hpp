@@ -3,6 +3,7 @@
 #include "../../include/wt.h"
 #include "lbm.hpp"
 #include "mlp.hpp"
+#include <vector>
 
 namespace wt {
 
@@ -29,13 +30,19 @@ void launch_classify(const LaunchCtx& c, uint32_t* list, uint32_t* masks, unsign
 void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* masks, WallLink* links, uint32_t* meta, int64_t count,
                      int64_t cap);
 // chunk: items per XCD run (<= 0: one contiguous eighth of the items per XCD)
+// Role table of a step launch (StepArgs::role) for the layout in A: wall group k (8 x BLOCK wall
+// cells, the first in row wall_rows[k]) goes right after the bulk group that updates row
+// wall_rows[k] + ny + 1, by when the bulk blocks have streamed the rows its links read (the cell's
+// own row and its y / z neighbours).
+constexpr int WALL_GROUP_CELLS = 8 * 256;  // wall cells per group of 8 wall blocks (the step kernels' 256-thread blocks)
+std::vector<uint32_t> build_roles(const StepArgs& A, const std::vector<uint32_t>& wall_rows, int ny, int nz);
 StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* meta, const WallLink* links, int64_t count, int64_t cap,
                    int64_t chunk);
 // ev0 / ev1 (may be null): events recorded at the step kernel's own start and end
 void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs& A, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // One step of a batch of domains sharing lattice, storage, cells per thread and collision.
 void launch_step_batch(hipStream_t st, int q, int prec, int cpt, bool reg, const DomainDesc* descs, const uint32_t* dmap,
-                       uint32_t wall_total, uint32_t blocks, int parity);
+                       uint32_t blocks, int parity);
 int cells_per_thread(const KParams& P);
 int64_t macro_blocks(int64_t n);
 void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, float* meanB, double* partials, double* result);

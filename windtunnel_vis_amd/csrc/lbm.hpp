@@ -275,17 +275,22 @@ struct StepArgs {
   uint32_t pad;
   FastDiv dwpr;     // division by wpr
   FastDiv dxk;      // division by xk
+  // role of each group of 8 launch blocks (one block per XCD), in launch order: bit 31 set: wall
+  // group (its blocks are wall blocks 8k..8k+7, k in the low bits), else bulk group (bulk blocks
+  // 8g..8g+7); (nwb + nbb) / 8 entries
+  const uint32_t* role;
 };
 
 // One domain of a batched step (k_step_batch): its parameters, launch layout, buffers and where its
-// blocks sit in the batched grid (all domains' wall blocks first, then all bulk blocks).
+// blocks sit in the batched grid (each domain's nwb + nbb blocks in a contiguous run, ordered by its
+// role table).
 struct DomainDesc {
   KParams P;
   StepArgs A;
   const void* f[2];
   const uint8_t* kind;
-  uint32_t wall_first;  // first wall block of the domain
-  uint32_t bulk_first;  // first bulk block of the domain (a multiple of 8)
+  uint32_t first;  // first block of the domain (a multiple of 8)
+  uint32_t pad;
 };
 
 // wgslLbm.ts:21-28

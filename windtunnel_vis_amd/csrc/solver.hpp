@@ -71,6 +71,14 @@ struct wt_solver {
   wt::MlpDev* mlp = nullptr;        // network in the MFMA layout (device)
   int64_t* pull = nullptr;          // bulk pull table (device), rebuilt with the parameters
   std::vector<int64_t> h_pull;      // its host copy
+  // placement of the wall blocks among the bulk blocks (StepArgs::role): the row of the first cell
+  // of every group of 8 x BLOCK wall cells (the wall list is sorted by cell), and the device table
+  // with the launch layout it was built for
+  std::vector<uint32_t> h_wrows;
+  uint64_t wall_version = 0;        // bumped whenever the wall list is rebuilt
+  uint32_t* role = nullptr;
+  size_t role_cap = 0;
+  uint64_t role_key[6] = {};
   // asynchronous statistics reads (wt_encode_post / wt_flush_reads): a ring of pinned slots, each
   // filled by a copy on the handle's stream and marked by an event, like the reference's staging
   // buffers (SolverGPU.ts:86-88, :249-257)
@@ -110,6 +118,7 @@ void derive(wt_solver* s);
 int run_init(wt_solver* s);
 int prep_walls(wt_solver* s);
 int reclassify(wt_solver* s);
+int ensure_roles(wt_solver* s, wt::StepArgs& A);
 int do_steps(wt_solver* s, int steps);
 int do_sample(wt_solver* s);
 void decode_stats(const wt_solver* s, wt_stats* out);

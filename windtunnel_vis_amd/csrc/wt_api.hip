@@ -107,6 +107,28 @@ int reclassify(wt_solver* s) {
   s->nwall = (int64_t)h[0];
   s->nfast = (int64_t)h[1];
   s->nsolid = (int64_t)h[2];
+  // the wall list in cell order (the collection appends wave by wave in any order), so that each
+  // group of wall blocks covers a short range of rows and can be placed beside them (build_roles)
+  s->h_wrows.clear();
+  if (s->nwall > 0) {
+    const size_t nw = (size_t)s->nwall;
+    std::vector<uint32_t> cells(nw), masks(nw), order(nw);
+    HIP_TRY(hipMemcpyAsync(cells.data(), s->wlist, 4 * nw, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(masks.data(), s->wmask, 4 * nw, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < nw; i++) order[i] = (uint32_t)i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cells[a] < cells[b]; });
+    std::vector<uint32_t> sc(nw), sm(nw);
+    for (size_t i = 0; i < nw; i++) {
+      sc[i] = cells[order[i]];
+      sm[i] = masks[order[i]];
+    }
+    HIP_TRY(hipMemcpyAsync(s->wlist, sc.data(), 4 * nw, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->wmask, sm.data(), 4 * nw, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < nw; i += WALL_GROUP_CELLS) s->h_wrows.push_back(sc[i] / (uint32_t)s->dims.nx);
+  }
+  s->wall_version++;
   if (s->nwall > s->wcap || !s->wlink) {
     if (s->wlink) HIP_TRY(hipFree(s->wlink));
     if (s->wmeta) HIP_TRY(hipFree(s->wmeta));
@@ -122,6 +144,25 @@ int reclassify(wt_solver* s) {
   return prep_walls(s);
 }
 
+int ensure_roles(wt_solver* s, StepArgs& A) {
+  const uint64_t key[6] = {A.nwb, A.nbb, A.xk, A.wpr, A.nwaves, s->wall_version};
+  if (!s->role || std::memcmp(key, s->role_key, sizeof(key)) != 0) {
+    const std::vector<uint32_t> tab = build_roles(A, s->h_wrows, s->dims.ny, s->dims.nz);
+    if (tab.size() > s->role_cap) {
+      if (s->role) HIP_TRY(hipFree(s->role));
+      s->role = nullptr;
+      s->role_cap = 0;
+      HIP_TRY(hipMalloc((void**)&s->role, 4 * tab.size()));
+      s->role_cap = tab.size();
+    }
+    HIP_TRY(hipMemcpyAsync(s->role, tab.data(), 4 * tab.size(), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::memcpy(s->role_key, key, sizeof(key));
+  }
+  A.role = s->role;
+  return WT_OK;
+}
+
 int do_steps(wt_solver* s, int steps) {
   if (s->pending_init) {
     int r = run_init(s);
@@ -131,7 +172,8 @@ int do_steps(wt_solver* s, int steps) {
     if (int r = ensure_wdist(s)) return r;
   }
   LaunchCtx c = s->ctx();
-  const StepArgs A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->item_chunk);
+  StepArgs A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->item_chunk);
+  if (int r = ensure_roles(s, A)) return r;
   // next free event pair, covering `launches` step launches
   auto pair = [&](int64_t launches, hipEvent_t*& e) -> int {
     if (s->ev_used + 2 > s->ev.size()) {
@@ -205,7 +247,7 @@ struct wt_batch {
   uint32_t* d_map = nullptr;
   int64_t map_cap = 0;
   std::vector<DomainDesc> h_desc;
-  uint32_t wall_total = 0, blocks = 0;
+  uint32_t blocks = 0;
   int q = 0, prec = 0, cpt = 0;
   bool reg = true;
   int chunk = 100;      // steps per captured graph (even)
@@ -243,8 +285,7 @@ int batch_member_check(const wt_solver* s, int q, int prec, int cpt, bool reg) {
 // captured graph when anything changed.
 int batch_prepare(wt_batch* b) {
   std::vector<DomainDesc> desc(b->m.size());
-  uint32_t wall = 0;
-  std::vector<uint32_t> nbb(b->m.size());
+  uint32_t blocks = 0;
   for (size_t k = 0; k < b->m.size(); k++) {
     if (int r = batch_member_check(b->m[k], b->q, b->prec, b->cpt, b->reg)) return r;
   }
@@ -255,27 +296,20 @@ int batch_prepare(wt_batch* b) {
     std::memset(&d, 0, sizeof(d));
     d.P = s->P;
     d.A = step_args(c, s->wlist, s->wmeta, s->wlink, s->nwall, s->wcap, s->item_chunk);
+    if (int r = ensure_roles(s, d.A)) return r;
     d.f[0] = s->f[0];
     d.f[1] = s->f[1];
     d.kind = s->kind;
-    d.wall_first = wall;
-    wall += d.A.nwb;  // multiples of 8: bulk blocks stay XCD-aligned
-    nbb[k] = d.A.nbb;
-  }
-  uint32_t bulk = wall;
-  for (size_t k = 0; k < b->m.size(); k++) {
-    desc[k].bulk_first = bulk;
-    bulk += nbb[k];
+    d.first = blocks;
+    blocks += d.A.nwb + d.A.nbb;  // multiples of 8: every domain's blocks keep their XCD
   }
   const bool same = desc.size() == b->h_desc.size() &&
                     std::memcmp(desc.data(), b->h_desc.data(), desc.size() * sizeof(DomainDesc)) == 0;
   if (same && b->d_desc) return WT_OK;
   batch_drop_graph(b);
-  std::vector<uint32_t> map(bulk);
-  for (size_t k = 0; k < b->m.size(); k++) {
-    for (uint32_t i = 0; i < desc[k].A.nwb; i++) map[desc[k].wall_first + i] = (uint32_t)k;
-    for (uint32_t i = 0; i < nbb[k]; i++) map[desc[k].bulk_first + i] = (uint32_t)k;
-  }
+  std::vector<uint32_t> map(blocks);
+  for (size_t k = 0; k < b->m.size(); k++)
+    for (uint32_t i = 0; i < desc[k].A.nwb + desc[k].A.nbb; i++) map[desc[k].first + i] = (uint32_t)k;
   if (!b->d_desc) HIP_TRY(hipMalloc((void**)&b->d_desc, sizeof(DomainDesc) * b->m.size()));
   if ((int64_t)map.size() > b->map_cap) {
     if (b->d_map) HIP_TRY(hipFree(b->d_map));
@@ -287,13 +321,12 @@ int batch_prepare(wt_batch* b) {
   HIP_TRY(hipMemcpyAsync(b->d_map, map.data(), sizeof(uint32_t) * map.size(), hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->h_desc = desc;
-  b->wall_total = wall;
-  b->blocks = bulk;
+  b->blocks = blocks;
   return WT_OK;
 }
 
 void batch_launch(wt_batch* b, int parity) {
-  launch_step_batch(b->stream, b->q, b->prec, b->cpt, b->reg, b->d_desc, b->d_map, b->wall_total, b->blocks, parity);
+  launch_step_batch(b->stream, b->q, b->prec, b->cpt, b->reg, b->d_desc, b->d_map, b->blocks, parity);
 }
 
 }  // namespace
@@ -377,7 +410,7 @@ int wt_destroy(wt_solver* s) {
   if (!s) return WT_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto e : s->ev) (void)hipEventDestroy(e);
-  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->wdist, s->mlp, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result, s->vort, s->stat, s->meanh, s->pull};
+  void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->wdist, s->mlp, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result, s->vort, s->stat, s->meanh, s->pull, s->role};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->h_result) (void)hipHostFree(s->h_result);
