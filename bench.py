@@ -5,7 +5,7 @@ Bouzidi walls, moving belt at u = 0.08 with rolling wheels, Cs = 0.1, around the
 (rounded box 85x22x32 cells, corner radius 6, four wheels r = 6, width 5, ride height 3, nose at
 0.34 nx). At this size the 2.5 GB ping-pong is ten times the 256 MB memory-side
 cache, so the HBM roofline fraction is the memory system's, not the cache's. One step = one fused
-pull-stream + collide pass over the whole grid, one launch of the step kernel: its first blocks
+pull-stream + collide pass over the whole grid, one launch of the step kernel: some of its blocks
 update the cells next to the body from precomputed links, the rest stream the bulk.
 
 Prints ONE JSON line. `value` is MLUPS over the timed steps with the populations resident in HBM.

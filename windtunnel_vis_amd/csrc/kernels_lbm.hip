@@ -659,9 +659,9 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
   for (int i = 0; i < Q; i++) S::st_dev(fout, base + i * nx, L::w(i), f[i]);
 }
 
-// One lattice step in one launch. The first A.nwb blocks update the wall cells (short gathers on
-// few lanes: started first so their latency hides under the bulk work); every other block is 4
-// waves of the bulk update, one item per wave. Items are numbered (row, chunk) with the chunk
+// One lattice step in one launch. A.nwb blocks update the wall cells (one lane per cell, gathers
+// from the precomputed links), placed among the bulk blocks beside the rows they read (block_role);
+// every other block is 4 waves of the bulk update, one item per wave. Items are numbered (row, chunk) with the chunk
 // fastest. Bulk block b runs on XCD b % 8 (blocks are dealt round-robin to the 8 XCDs); the items
 // are cut into runs of A.xk consecutive items, the runs dealt round-robin to the XCDs, so that
 // the cache lines shared by neighbouring rows of one run stay inside one L2. A.xk = 4 is the plain

@@ -259,7 +259,8 @@ __host__ __device__ inline void pull_source(int i, bool yLo, bool yHi, bool zLo,
   dx = grnd ? 0 : -cx;
 }
 
-// Launch layout of one step (see k_step): wall blocks first, then bulk blocks of 4 waves.
+// Launch layout of one step (see k_step): wall blocks and bulk blocks of 4 waves, in groups of 8
+// ordered by the role table.
 struct StepArgs {
   const int64_t* pull;     // pull table: (PULL_CLASSES + 1) x Q element offsets (see pull_source)
   const uint32_t* wlist;   // wall cells
