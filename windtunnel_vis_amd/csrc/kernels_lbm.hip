@@ -666,7 +666,10 @@ __device__ __forceinline__ void wall_cell(const typename S::T* __restrict__ fin,
 // are cut into runs of A.xk consecutive items, the runs dealt round-robin to the XCDs, so that
 // the cache lines shared by neighbouring rows of one run stay inside one L2. A.xk = 4 is the plain
 // order (block b takes items 4b..4b+3); A.xk = nwaves / 8 gives every XCD one contiguous eighth.
-template <class S> constexpr int step_waves() { return S::BYTES == 2 ? 6 : 4; }
+// Occupancy of the plain step: five waves per SIMD for the 16-bit formats measured faster than six
+// (419.7 vs 426.5 us at 512x256x256, 57.9 vs 59.7 us at 256x128x128) and than four (425) or seven
+// (436, with spills).
+template <class S> constexpr int step_waves() { return S::BYTES == 2 ? 5 : 4; }
 // The batched step reads its parameters and buffers from the descriptor table, which costs it
 // registers: at six waves per SIMD its 16-bit forms spilled (32 bytes per lane); five waves keep
 // them spill-free and measured faster on the 24-domain sweep (61.2 vs 60.4 k MLUPS).
