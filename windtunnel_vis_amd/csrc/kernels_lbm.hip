@@ -289,8 +289,19 @@ struct BulkIn {
 struct RowPos {
   uint32_t row;
   int iy, iz, x0;
-  const int64_t* pull;  // the pull table (source offsets per row class, then i * n)
+  const int64_t* pull;  // the pull table (source offsets per row class)
 };
+
+// A wave-uniform base address kept opaque to the optimiser, so that the per-lane 32-bit byte offset
+// is added by the memory instruction itself (global_load / global_store with a scalar base and a
+// vector offset) instead of by 64-bit vector arithmetic per access, which the compiler otherwise
+// emits after folding the lane offset into the sum first. Only for values the compiler holds in
+// scalar registers (kernel arguments and scalar-loaded tables, not tables read after a global store).
+template <class T>
+__device__ __forceinline__ T* sgpr_base(T* p) {
+  asm("" : "+s"(p));
+  return p;
+}
 
 __device__ __forceinline__ RowPos item_pos(uint32_t item, const StepArgs& A, const KParams& P, int cpt) {
   RowPos r;
@@ -304,7 +315,7 @@ __device__ __forceinline__ RowPos item_pos(uint32_t item, const StepArgs& A, con
   return r;
 }
 
-template <class L, class S, int CPT, bool MLP = false>
+template <class L, class S, int CPT, bool MLP = false, bool SADDR = false>
 __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin, const uint8_t* __restrict__ kind,
                                           const KParams& P, const RowPos& r, BulkIn<L, S, CPT>& in,
                                           const uint8_t* __restrict__ wdist = nullptr) {
@@ -342,7 +353,7 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
       xlim[i] = (grnd || cx >= 0) ? NX - 1 : NX - 2;
     }
   }
-  // all loads in flight together (scalar base + 32-bit lane byte offset)
+  // all loads in flight together
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
   if constexpr (CPT == 2) {
     // The x-shifted word of the row's first pair (c_x > 0: cells -1, 0) and of its last pair
@@ -351,26 +362,55 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
     // used (the inlet replaces it, the outlet clamp reads the other half), so those two lanes read
     // the aligned word of their own pair instead and move the needed half into place (measured:
     // 4 % less read traffic at nx = 256, 5 % at nx = 512).
-    // Only waves holding a row end take that path (wave-uniform branch).
     using Word = typename S::Word;
     constexpr int HALF = 4 * (int)sizeof(Word);
     const int wx = __builtin_amdgcn_readfirstlane(x0);  // the wave's first x (lane 0)
-    if (wx == 0 || wx + 128 >= NX) {
-      const bool first = x0 == 0, last = x0 + 2 == NX;
+    const bool first = x0 == 0, last = x0 + 2 == NX;
+    if constexpr (SADDR) {
+      // Scalar bases and 32-bit lane offsets (no 64-bit vector address arithmetic): the bases lie 4
+      // bytes before the sources and the lane offsets 4 bytes further on, because with nx = 2 the
+      // last pair starts at x0 = 0 and its aligned word lies one element before the shifted source,
+      // which a non-negative offset from the shifted source could not reach. The lane offsets cost
+      // a few instructions per wave; the half moves are taken only by waves holding a row end
+      // (wave-uniform branch after the loads).
+      const uint32_t o_mid = lane_off + 4u;
+      const uint32_t o_pos = o_mid + (first ? (uint32_t)sizeof(T) : 0u);  // c_x > 0
+      const uint32_t o_neg = o_mid - (last ? (uint32_t)sizeof(T) : 0u);   // c_x < 0
 #pragma unroll
       for (int i = 0; i < Q; i++) {
         const int cx = L::c(i, 0), cy = L::c(i, 1);
         const bool grnd = cy > 0 && yLo && P.ground != 0;  // bounce-back: no x shift
-        const bool edge = cx != 0 && !grnd && (cx > 0 ? first : last);
-        // signed: with nx = 2 the last pair starts at x0 = 0 and its aligned word lies one
-        // element before the shifted source (lane_off + c_x * sizeof(T) < 0)
-        const int32_t o = (int32_t)lane_off + (edge ? cx * (int32_t)sizeof(T) : 0);
-        const Word v = S::ld_word((const T*)((const char*)src[i] + o));
-        in.raw[i] = !edge ? v : (cx > 0 ? (Word)(v << HALF) : (Word)(v >> HALF));
+        const uint32_t o = (cx == 0 || grnd) ? o_mid : cx > 0 ? o_pos : o_neg;
+        in.raw[i] = S::ld_word((const T*)(sgpr_base((const char*)src[i] - 4) + o));
+      }
+      if (wx == 0 || wx + 128 >= NX) {
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+          const int cx = L::c(i, 0), cy = L::c(i, 1);
+          const bool grnd = cy > 0 && yLo && P.ground != 0;
+          const bool edge = cx != 0 && !grnd && (cx > 0 ? first : last);
+          if (cx != 0 && edge) in.raw[i] = cx > 0 ? (Word)(in.raw[i] << HALF) : (Word)(in.raw[i] >> HALF);
+        }
       }
     } else {
+      // 64-bit vector addresses; only waves holding a row end take the shifted-pair path
+      // (wave-uniform branch)
+      if (wx == 0 || wx + 128 >= NX) {
 #pragma unroll
-      for (int i = 0; i < Q; i++) in.raw[i] = S::ld_word((const T*)((const char*)src[i] + lane_off));
+        for (int i = 0; i < Q; i++) {
+          const int cx = L::c(i, 0), cy = L::c(i, 1);
+          const bool grnd = cy > 0 && yLo && P.ground != 0;  // bounce-back: no x shift
+          const bool edge = cx != 0 && !grnd && (cx > 0 ? first : last);
+          // signed: with nx = 2 the last pair starts at x0 = 0 and its aligned word lies one
+          // element before the shifted source (lane_off + c_x * sizeof(T) < 0)
+          const int32_t o = (int32_t)lane_off + (edge ? cx * (int32_t)sizeof(T) : 0);
+          const Word v = S::ld_word((const T*)((const char*)src[i] + o));
+          in.raw[i] = !edge ? v : (cx > 0 ? (Word)(v << HALF) : (Word)(v >> HALF));
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < Q; i++) in.raw[i] = S::ld_word((const T*)((const char*)src[i] + lane_off));
+      }
     }
   } else {
 #pragma unroll
@@ -392,8 +432,10 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
   constexpr uint32_t SOLID_ALL = CPT == 2 ? 0x0101u : 0x01u;
   constexpr uint32_t SKIP = CPT == 2 ? 0x0303u : 0x03u;
   const uint32_t lane_off = (uint32_t)x0 * (uint32_t)sizeof(T);
+  // (plain 64-bit address arithmetic here: with scalar bases the compiler contracted the batched
+  // step's collision differently from this kernel's, and the two must agree bit for bit)
   auto store = [&](int i, float a, float b) {
-    T* o = (T*)((char*)(fout + rowoff + glob(r.pull)[PULL_CLASSES * Q + i]) + lane_off);
+    T* o = (T*)((char*)(fout + rowoff + (int64_t)i * NX) + lane_off);
     if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
     else S::st_dev(o, 0, L::w(i), a);
   };
@@ -491,8 +533,7 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
 template <class L, class S, int CPT, bool REG>
 __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                                                 const uint8_t* __restrict__ kind, const KParams& P, const RowPos& r,
-                                                const BulkIn<L, S, CPT>& in0, const MlpDev* __restrict__ net, short* feats,
-                                                float* nus) {
+                                                const BulkIn<L, S, CPT>& in0, const MlpDev* __restrict__ net) {
   using T = typename S::T;
   using In = BulkIn<L, S, CPT>;
   constexpr int Q = L::Q;
@@ -541,35 +582,33 @@ __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict_
     for (int i = 0; i < Q; i++) f[i] = pull_b(in, i);
     return relax<L>(f, (float)(x0 + 1), gwall, reg, P);
   };
-  // inputs of both cells, normalised, staged as BF16 for the wave's MFMA evaluation. Cells whose
-  // result is not used (solid, wall-path or past the row end) stage their inputs unmasked: a cell
-  // is one column of every product, so its inputs reach no other cell's output; a non-finite input
-  // of a live cell gives a non-finite output, which keeps Smagorinsky.
-  const int lane = threadIdx.x & 63;
-  auto stage = [&](const Relax& m, uint32_t wdb, int slot) {
+  // inputs of both cells, normalised and packed as BF16 pairs for the wave's MFMA evaluation.
+  // Cells whose result is not used (solid, wall-path or past the row end) take part unmasked: a
+  // cell is one column of every product, so its inputs reach no other cell's output; a non-finite
+  // input of a live cell gives a non-finite output, which keeps Smagorinsky.
+  auto inputs = [&](const Relax& m, uint32_t wdb, uint32_t (&x)[MLP_IN / 2]) {
     // S = -3 Pi / (2 rho tau) = sc (-Pi), sc = 1.5 omega / rho = 6.75 omega / (4.5 rho)
     const float sc = 6.75f * m.omega * fast_rcp(m.rho45);
     float v[MLP_IN];
     mlp_step_inputs(m.pxx, m.pyy, m.pzz, m.pxy, m.pxz, m.pyz, sc, wdb, fmaf(m.ux, m.ux, fmaf(m.uy, m.uy, m.uz * m.uz)), P.inu32,
                     net->norm, v);
-    uint32_t* dst = (uint32_t*)(feats + slot * MLP_IN);
 #pragma unroll
-    for (int i = 0; i < MLP_IN / 2; i++) dst[i] = pack_bf16(v[2 * i], v[2 * i + 1]);
+    for (int i = 0; i < MLP_IN / 2; i++) x[i] = pack_bf16(v[2 * i], v[2 * i + 1]);
   };
+  // cell a through the network, then cell b (only one cell's inputs live at a time)
   Relax ma = moments_a(in0, reg_row), mb;
-  stage(ma, in0.wd & 0xffu, 2 * lane);
-  if constexpr (CPT == 2) {
-    mb = moments_b(in0, reg_row);
-    stage(mb, (in0.wd >> 8) & 0xffu, 2 * lane + 1);
-  } else {
-    stage(ma, 0u, 2 * lane + 1);
+  if constexpr (CPT == 2) mb = moments_b(in0, reg_row);
+  float nu_a, nu_b = 0.0f;
+  {
+    uint32_t x[MLP_IN / 2];
+    inputs(ma, in0.wd & 0xffu, x);
+    nu_a = mlp_nu_of(net, mlp_wave_z(net, x));
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  mlp_wave_nu(net, feats, nus);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const float nu_a = nus[2 * lane], nu_b = nus[2 * lane + 1];
+  if constexpr (CPT == 2) {
+    uint32_t x[MLP_IN / 2];
+    inputs(mb, (in0.wd >> 8) & 0xffu, x);
+    nu_b = mlp_nu_of(net, mlp_wave_z(net, x));
+  }
   if (x0 >= NX || (k != SOLID_ALL && (k & SKIP))) return;
   // opaque copies of the source and pull-table pointers, so that the compiler neither merges the
   // BGK rows' second loads with the first ones nor keeps the first ones' values live through the
@@ -578,7 +617,7 @@ __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict_
   RowPos r2 = r;
   asm volatile("" : "+s"(f2), "+s"(r2.pull));
   auto store = [&](int i, float a, float b) {
-    T* o = (T*)((char*)(fout + rowoff + glob(r2.pull)[PULL_CLASSES * Q + i]) + lane_off);
+    T* o = (T*)((char*)sgpr_base(fout + rowoff + (int64_t)i * NX) + lane_off);
     if constexpr (CPT == 2) S::st2_dev(o, L::w(i), a, b);
     else S::st_dev(o, 0, L::w(i), a);
   };
@@ -587,12 +626,20 @@ __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict_
     for (int i = 0; i < Q; i++) store(i, 0.0f, 0.0f);
     return;
   }
+  // t0 and the inlet-layer weight again from x (cheaper than holding them through the network; the
+  // opaque position keeps the compiler from reusing the first values)
+  {
+    float xa = (float)x0;
+    asm volatile("" : "+v"(xa));
+    x_terms(xa, gwall, P, ma.t0, ma.sIn);
+    if constexpr (CPT == 2) x_terms(xa + 1.0f, gwall, P, mb.t0, mb.sIn);
+  }
   // nu_t = max(network, floor x nu_smag), nu_smag = (tau_smag - t0) / 3; non-finite: Smagorinsky
   const float floor_ = net->norm.smag_floor;
   auto finish = [&](auto rg) {
     constexpr bool reg = decltype(rg)::value;
     In in;
-    if constexpr (!reg) bulk_load<L, S, CPT>(f2, kind, P, r2, in);
+    if constexpr (!reg) bulk_load<L, S, CPT, false, false>(f2, kind, P, r2, in);
     else in = in0;  // unused: the regularized outputs follow from the moments
     // t0 + 3 max(nu, floor nu_smag) = max(t0 + 3 nu, t0 + floor (tau_smag - t0)); the comparison is
     // false for NaN as well
@@ -678,8 +725,7 @@ template <class S> constexpr int batch_waves() { return S::BYTES == 2 ? 5 : 4; }
 template <class L, class S, int CPT, bool REG, bool MLP = false>
 __device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                                            const uint8_t* __restrict__ kind, const KParams& P, const StepArgs& A, uint32_t b,
-                                           const uint8_t* __restrict__ wdist = nullptr, const MlpDev* __restrict__ net = nullptr,
-                                           short* feats = nullptr, float* nus = nullptr) {
+                                           const uint8_t* __restrict__ wdist = nullptr, const MlpDev* __restrict__ net = nullptr) {
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lw = (b / 8u) * 4u + w;  // wave index on this XCD
   const uint32_t m = A.dxk.div(lw);
@@ -687,9 +733,11 @@ __device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin
   if (item >= A.nwaves) return;
   BulkIn<L, S, CPT> in;
   const RowPos r = item_pos(item, A, P, CPT);
-  bulk_load<L, S, CPT, MLP>(fin, kind, P, r, in, wdist);
+  // the learned step loads through scalar bases (fewer vector instructions where the vector pipe
+  // is the bound); the plain step keeps 64-bit vector addresses, measured faster at 512x256x256
+  bulk_load<L, S, CPT, MLP, MLP>(fin, kind, P, r, in, wdist);
   if constexpr (MLP) {
-    bulk_finish_mlp<L, S, CPT, REG>(fin, fout, kind, P, r, in, net, feats, nus);
+    bulk_finish_mlp<L, S, CPT, REG>(fin, fout, kind, P, r, in, net);
   } else {
     // ground rows relax with BGK (wall cells, wgslLbm.ts:186); the branch is wave-uniform
     if (REG && !(r.iy == 0 && P.ground != 0)) bulk_finish<L, S, CPT, true>(fout, P, r, in);
@@ -724,11 +772,9 @@ __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step(const typename 
 
 // The step with the learned closure in the bulk (D3Q19); wall cells keep Smagorinsky.
 template <class S, int CPT, bool REG>
-__global__ __launch_bounds__(BLOCK, 5) void k_step_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+__global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
                                                     const uint8_t* __restrict__ kind, const uint8_t* __restrict__ wdist,
                                                     const MlpDev* __restrict__ net, KParams P, StepArgs A) {
-  __shared__ short feats[BLOCK / 64][128 * MLP_IN];
-  __shared__ float nus[BLOCK / 64][128];
   __shared__ MlpDev snet;
   const BlockRole br = block_role(A, blockIdx.x);
   if (br.wall) {
@@ -736,8 +782,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_step_mlp(const typename S::T* __re
     return;
   }
   stage_net(net, &snet);  // bulk blocks only: the whole block passes here
-  const int w = threadIdx.x >> 6;
-  bulk_block<D3Q19, S, CPT, REG, true>(fin, fout, kind, P, A, br.index, wdist, &snet, feats[w], nus[w]);
+  bulk_block<D3Q19, S, CPT, REG, true>(fin, fout, kind, P, A, br.index, wdist, &snet);
 }
 
 // One lattice step of many independent domains in one launch (sweeps): the same wall and bulk
@@ -1053,7 +1098,7 @@ void launch_wallprep(const LaunchCtx& c, const uint32_t* list, const uint32_t* m
                        meta, count, cap, c.P, *c.W);
 }
 
-int pull_table_len(int q) { return (PULL_CLASSES + 1) * q; }
+int pull_table_len(int q) { return PULL_CLASSES * q; }
 
 void pull_table(const KParams& P, int64_t* out) {
   auto go = [&](auto l) {
@@ -1066,7 +1111,6 @@ void pull_table(const KParams& P, int64_t* out) {
         // the source cell's row is row + dy + dz * ny; its block starts (dy + dz * ny) * Q * nx later
         out[cls * L::Q + i] = (((int64_t)dy + (int64_t)dz * P.ny) * L::Q + d) * nx + dx;
       }
-    for (int i = 0; i < L::Q; i++) out[PULL_CLASSES * L::Q + i] = (int64_t)i * nx;
   };
   if (P.q == 9) go(D2Q9{});
   else go(D3Q19{});

@@ -421,30 +421,29 @@ void launch_adam(hipStream_t st, float* p, float* m, float* v, const float* g, f
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_mlp_infer(const MlpDev* __restrict__ gnet, const float* __restrict__ raw, int64_t n,
                                                      float* __restrict__ nu) {
-  __shared__ short feats[BLOCK / 64][128 * MLP_IN];
-  __shared__ float res[BLOCK / 64][128];
   __shared__ MlpDev snet;
   stage_net(gnet, &snet);
   const MlpDev* net = &snet;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int64_t base = ((int64_t)blockIdx.x * (BLOCK / 64) + w) * 128;
-  // every lane stages two cells (2l, 2l + 1), as the step kernel's lanes do
+  // every lane evaluates two cells (2l, 2l + 1), as the step kernel's lanes do
+  uint32_t x[2][MLP_IN / 2];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const int64_t c = base + 2 * l + h;
+    float v[MLP_IN];
 #pragma unroll
-    for (int i = 0; i < MLP_IN; i++) {
-      const float x = c < n ? (raw[c * MLP_IN + i] - net->norm.mu[i]) * net->norm.isig[i] : 0.0f;
-      feats[w][(2 * l + h) * MLP_IN + i] = to_bf16(x);
-    }
+    for (int i = 0; i < MLP_IN; i++) v[i] = c < n ? (raw[c * MLP_IN + i] - net->norm.mu[i]) * net->norm.isig[i] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < MLP_IN / 2; i++) x[h][i] = (uint32_t)(uint16_t)to_bf16(v[2 * i]) | ((uint32_t)(uint16_t)to_bf16(v[2 * i + 1]) << 16);
   }
-  __syncthreads();
-  mlp_wave_nu(net, feats[w], res[w]);
-  __syncthreads();
+  float r[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) r[h] = mlp_nu_of(net, mlp_wave_z(net, x[h]));
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const int64_t c = base + 2 * l + h;
-    if (c < n) nu[c] = res[w][2 * l + h];
+    if (c < n) nu[c] = r[h];
   }
 }
 

@@ -237,8 +237,8 @@ enum : uint32_t { WM_SOLID = 1u, WM_WALL = 2u, WM_GWALL = 4u };
 // pulls from a fixed source direction and a fixed (dy, dz, dx) shift, which the face rules of the
 // pull step decide (wgslLbm.ts:139-170: ground bounce-back, y-z edge mirror, y mirror, z mirror or
 // periodic wrap, plain pull). The pull table holds, per class and direction, the element offset of
-// the pulled value relative to the pulling cell (d * n + dy * nx + dz * nx * ny + dx), then one
-// more row with i * n (the stores). Built on the host (pull_table) for each parameter set.
+// the pulled value relative to the pulling cell's row block (row-blocked layout: ((dy + dz ny) Q +
+// d) nx + dx). Built on the host (pull_table) for each parameter set.
 constexpr int PULL_CLASSES = 16;
 
 template <class L>
@@ -262,7 +262,7 @@ __host__ __device__ inline void pull_source(int i, bool yLo, bool yHi, bool zLo,
 // Launch layout of one step (see k_step): wall blocks and bulk blocks of 4 waves, in groups of 8
 // ordered by the role table.
 struct StepArgs {
-  const int64_t* pull;     // pull table: (PULL_CLASSES + 1) x Q element offsets (see pull_source)
+  const int64_t* pull;     // pull table: PULL_CLASSES x Q element offsets (see pull_source)
   const uint32_t* wlist;   // wall cells
   const uint32_t* wmeta;   // their meta words
   const WallLink* wlink;   // their links, direction-major: wlink[(i - 1) * wcap + t]
@@ -364,6 +364,16 @@ struct Relax {
   float E0, trP;       // E = E0 - k1 trP (regularized) or E0 (BGK)
 };
 
+// The terms of a cell's update that depend only on its x position: the relaxation time before any
+// closure (tau0 + outlet sponge, floored on ground-wall rows) and the inlet absorbing-layer weight.
+__device__ __forceinline__ void x_terms(float xpos, bool gwall, const KParams& P, float& t0, float& sIn) {
+  const float sp = __builtin_amdgcn_fmed3f((xpos - P.spongeStart) * P.invSponge, 0.0f, 1.0f);
+  t0 = P.tau0 + 3.0f * P.spongeNu * sp * sp;
+  if (gwall) t0 = fmaxf(t0, P.tauWall);
+  const float si = __builtin_amdgcn_fmed3f(1.0f - xpos * P.invSpongeIn, 0.0f, 1.0f);
+  sIn = P.spongeIn > 0.0f ? 0.08f * si * si : 0.0f;
+}
+
 template <class L>
 __device__ __forceinline__ Relax relax(const float (&d)[L::Q], float xpos, bool gwall, bool reg, const KParams& P) {
   constexpr int Q = L::Q;
@@ -409,9 +419,8 @@ __device__ __forceinline__ Relax relax(const float (&d)[L::Q], float xpos, bool 
   const float pxz = D3 ? sxz - jx * uz : 0.0f;
   const float pyz = D3 ? syz - jy * uz : 0.0f;
   const float usq = D3 ? ux * ux + uy * uy + uz * uz : ux * ux + uy * uy;
-  const float sp = __builtin_amdgcn_fmed3f((xpos - P.spongeStart) * P.invSponge, 0.0f, 1.0f);
-  float t0 = P.tau0 + 3.0f * P.spongeNu * sp * sp;
-  if (gwall) t0 = fmaxf(t0, P.tauWall);
+  float t0, sIn;
+  x_terms(xpos, gwall, P, t0, sIn);
   float tau = t0;
   if (P.smag > 0.0f) {
     const float pn2 = D3 ? pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz)
@@ -422,8 +431,7 @@ __device__ __forceinline__ Relax relax(const float (&d)[L::Q], float xpos, bool 
   }
   const float omega = fast_rcp(tau);
   const float om1 = 1.0f - omega;
-  const float si = __builtin_amdgcn_fmed3f(1.0f - xpos * P.invSpongeIn, 0.0f, 1.0f);
-  m.sIn = P.spongeIn > 0.0f ? 0.08f * si * si : 0.0f;
+  m.sIn = sIn;
   m.rho45 = 4.5f * rho;
   m.ux = ux; m.uy = uy; m.uz = uz;
   m.jx = jx; m.jy = jy; m.jz = jz;

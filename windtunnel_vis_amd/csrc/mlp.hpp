@@ -181,10 +181,6 @@ struct alignas(16) MlpDev {
   MlpNorm norm;
 };
 
-// Evaluates nu_t for the 128 cells of a wave. net: the network (best an LDS copy, see
-// stage_net); feats: this wave's LDS staging area of 128 x 8 BF16 (cell-major; every lane wrote
-// its own cells before the call); nu: 128 floats of LDS for the results. All 64 lanes must call
-// it (MFMA uses the whole wave); cells are indexed 0..127.
 __device__ inline bf16x8_hw relu_pack8(const f32x16_t& acc, int c) {  // D registers 8c..8c+7 -> K = 16 B operand
   u32x4_hw w;
 #pragma unroll
@@ -192,38 +188,67 @@ __device__ inline bf16x8_hw relu_pack8(const f32x16_t& acc, int c) {  // D regis
   return __builtin_bit_cast(bf16x8_hw, w);
 }
 
-__device__ inline void mlp_wave_nu(const MlpDev* __restrict__ net, const short* feats, float* nu) {
+// Evaluates the network for 64 cells of a wave, one per lane (each lane holds two cells, a and b, and
+// calls this once per cell), with no staging of inputs or results in LDS. x: the lane's normalised
+// inputs of the cell as packed BF16 pairs (word j holds inputs 2j, 2j + 1). Group 0 of 32 columns
+// holds in column j the cell of lane j, group 1 that of lane 32 + j. The K = 8 B operand of a group
+// has lane half h hold inputs 4h..4h+3 of its column, so lane j < 32 needs inputs 0-3 and lane 32 + j
+// inputs 4-7 of the same cell: one v_permlane32_swap per word (lanes 32-63 of the first operand
+// trade places with lanes 0-31 of the second) turns (inputs 0-3, inputs 4-7) of every lane's cell
+// into the operands of groups 0 and 1. Every lane receives z of its column in every output register,
+// so lane half h takes group h: its own cell. All 64 lanes must call it (MFMA uses the whole wave).
+// net: the network, best an LDS copy (stage_net).
+__device__ inline bf16x4_t mlp_operand(uint32_t lo, uint32_t hi) {
+  typedef uint32_t u32x2_hw __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(bf16x4_t, (u32x2_hw){lo, hi});
+}
+
+__device__ inline float mlp_group_z(const MlpDev* __restrict__ net, bf16x4_t xb, int g) {
+  // the weights, biases and output weights are read from LDS in each group, each just before its
+  // product: held in registers across the groups (58 of them) they would set the register budget
+  // of the step kernel that fuses this evaluation; the opaque indices keep the compiler from
+  // hoisting the reads out of the groups
+  int l = (int)(threadIdx.x & 63) + 0 * g;
+  asm volatile("" : "+v"(l));
+  const int hg = l >> 5;
+  f32x16_t acc;
+#pragma unroll
+  for (int r = 0; r < 16; r++) acc[r] = net->b1d[hg][r];
+  acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(net->w1a[l], xb, acc, 0, 0, 0);
+  bf16x8_hw hb0 = relu_pack8(acc, 0), hb1 = relu_pack8(acc, 1);
+#pragma unroll
+  for (int r = 0; r < 16; r++) acc[r] = net->b2d[hg][r];
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w2b[0][l]), hb0, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w2b[1][l]), hb1, acc, 0, 0, 0);
+  // output layer on the matrix cores as well: w3 in every row of the A operand, so every register
+  // of every lane holds z of the lane's cell column
+  hb0 = relu_pack8(acc, 0);
+  hb1 = relu_pack8(acc, 1);
+  f32x16_t out = {};
+  out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w3b[0][hg]), hb0, out, 0, 0, 0);
+  out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w3b[1][hg]), hb1, out, 0, 0, 0);
+  return out[0];
+}
+
+// The network's output for one cell of every lane (x: its packed inputs): groups 2h' and 2h' + 1
+// for cells a and b are evaluated by two calls, cells a first, so that only one cell's operands are
+// live at a time. Returns z + b3 for the lane's own cell.
+__device__ inline float mlp_wave_z(const MlpDev* __restrict__ net, const uint32_t (&x)[4]) {
   const int l = threadIdx.x & 63;
-  const int h = l >> 5;
-  const bf16x4_t w1 = net->w1a[l];
-  const bf16x8_hw w2[2] = {__builtin_bit_cast(bf16x8_hw, net->w2b[0][l]), __builtin_bit_cast(bf16x8_hw, net->w2b[1][l])};
-  // one group of 32 cells at a time, the biases and output weights read from LDS in each pass:
-  // kept in registers across the groups (48 of them, plus interleaved accumulators) they would
-  // set the register budget of a kernel that fuses this evaluation
+  const auto s0 = __builtin_amdgcn_permlane32_swap(x[0], x[2], false, false);
+  const auto s1 = __builtin_amdgcn_permlane32_swap(x[1], x[3], false, false);
+  // one group at a time (not interleaved by the compiler, which would hold two groups' accumulators)
+  float z = 0.0f;
 #pragma unroll 1
-  for (int g = 0; g < 4; g++) {
-    const int cell = 32 * g + (l & 31);
-    const bf16x4_t xb = *(const bf16x4_t*)(feats + cell * MLP_IN + 4 * h);
-    int hg = h;
-    asm volatile("" : "+v"(hg));  // opaque per pass: the LDS reads below stay inside the loop
-    f32x16_t acc;
-#pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = net->b1d[hg][r];
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1, xb, acc, 0, 0, 0);
-    bf16x8_hw hb0 = relu_pack8(acc, 0), hb1 = relu_pack8(acc, 1);
-#pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = net->b2d[hg][r];
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2[0], hb0, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2[1], hb1, acc, 0, 0, 0);
-    // output layer on the matrix cores as well: w3 in every row of the A operand, so every register
-    // of every lane holds z of the lane's cell column
-    hb0 = relu_pack8(acc, 0);
-    hb1 = relu_pack8(acc, 1);
-    f32x16_t out = {};
-    out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w3b[0][hg]), hb0, out, 0, 0, 0);
-    out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w3b[1][hg]), hb1, out, 0, 0, 0);
-    if (h == 0) nu[cell] = net->norm.nu_scale * fminf(softplus(out[0] + net->b3), MLP_TARGET_CLIP);
+  for (int g = 0; g < 2; g++) {  // g = 0: the cells of lanes 0-31, g = 1: those of lanes 32-63
+    const float zg = mlp_group_z(net, mlp_operand(g ? s0[1] : s0[0], g ? s1[1] : s1[0]), g);
+    if ((l >= 32) == (g == 1)) z = zg;
   }
+  return z + net->b3;
+}
+
+__device__ inline float mlp_nu_of(const MlpDev* __restrict__ net, float z) {
+  return net->norm.nu_scale * fminf(softplus(z), MLP_TARGET_CLIP);
 }
 
 // Block-wide copy of the network into LDS (every thread of the block must call it).
