@@ -315,7 +315,30 @@ __device__ __forceinline__ RowPos item_pos(uint32_t item, const StepArgs& A, con
   return r;
 }
 
-template <class L, class S, int CPT, bool MLP = false, bool SADDR = false>
+// Row ends of a two-cells-per-lane item loaded through scalar bases (bulk_load with SADDR): the lanes
+// holding a row's first or last pair read their own aligned word for the x-shifted directions and
+// move the needed half into place here (wave-uniform branch: only waves holding a row end).
+template <class L, class S>
+__device__ __forceinline__ void bulk_load_fixup(const KParams& P, const RowPos& r, BulkIn<L, S, 2>& in) {
+  using Word = typename S::Word;
+  constexpr int HALF = 4 * (int)sizeof(Word);
+  const int NX = P.nx;
+  const int x0 = r.x0 < NX ? r.x0 : 0;
+  const bool yLo = r.iy == 0;
+  const bool first = x0 == 0, last = x0 + 2 == NX;
+  const int wx = __builtin_amdgcn_readfirstlane(x0);  // the wave's first x (lane 0)
+  if (wx == 0 || wx + 128 >= NX) {
+#pragma unroll
+    for (int i = 0; i < L::Q; i++) {
+      const int cx = L::c(i, 0), cy = L::c(i, 1);
+      const bool grnd = cy > 0 && yLo && P.ground != 0;
+      const bool edge = cx != 0 && !grnd && (cx > 0 ? first : last);
+      if (cx != 0 && edge) in.raw[i] = cx > 0 ? (Word)(in.raw[i] << HALF) : (Word)(in.raw[i] >> HALF);
+    }
+  }
+}
+
+template <class L, class S, int CPT, bool MLP = false, bool SADDR = false, bool FIXUP = true>
 __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin, const uint8_t* __restrict__ kind,
                                           const KParams& P, const RowPos& r, BulkIn<L, S, CPT>& in,
                                           const uint8_t* __restrict__ wdist = nullptr) {
@@ -383,15 +406,7 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
         const uint32_t o = (cx == 0 || grnd) ? o_mid : cx > 0 ? o_pos : o_neg;
         in.raw[i] = S::ld_word((const T*)(sgpr_base((const char*)src[i] - 4) + o));
       }
-      if (wx == 0 || wx + 128 >= NX) {
-#pragma unroll
-        for (int i = 0; i < Q; i++) {
-          const int cx = L::c(i, 0), cy = L::c(i, 1);
-          const bool grnd = cy > 0 && yLo && P.ground != 0;
-          const bool edge = cx != 0 && !grnd && (cx > 0 ? first : last);
-          if (cx != 0 && edge) in.raw[i] = cx > 0 ? (Word)(in.raw[i] << HALF) : (Word)(in.raw[i] >> HALF);
-        }
-      }
+      if (FIXUP) bulk_load_fixup<L, S>(P, r, in);
     } else {
       // 64-bit vector addresses; only waves holding a row end take the shifted-pair path
       // (wave-uniform branch)
@@ -722,27 +737,49 @@ template <class S> constexpr int step_waves() { return S::BYTES == 2 ? 5 : 4; }
 // them spill-free and measured faster on the 24-domain sweep (61.2 vs 60.4 k MLUPS).
 template <class S> constexpr int batch_waves() { return S::BYTES == 2 ? 5 : 4; }
 
-template <class L, class S, int CPT, bool REG, bool MLP = false>
-__device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
-                                           const uint8_t* __restrict__ kind, const KParams& P, const StepArgs& A, uint32_t b,
-                                           const uint8_t* __restrict__ wdist = nullptr, const MlpDev* __restrict__ net = nullptr) {
-  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+// The bulk item of wave w of launch block b (see k_step for the placement).
+__device__ __forceinline__ uint32_t bulk_item(const StepArgs& A, uint32_t b, uint32_t w) {
   const uint32_t lw = (b / 8u) * 4u + w;  // wave index on this XCD
   const uint32_t m = A.dxk.div(lw);
-  const uint32_t item = (m * 8u + b % 8u) * A.xk + (lw - m * A.xk);
+  return (m * 8u + b % 8u) * A.xk + (lw - m * A.xk);
+}
+
+template <class L, class S, int CPT, bool REG>
+__device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                           const uint8_t* __restrict__ kind, const KParams& P, const StepArgs& A, uint32_t b) {
+  const uint32_t item = bulk_item(A, b, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
   if (item >= A.nwaves) return;
   BulkIn<L, S, CPT> in;
   const RowPos r = item_pos(item, A, P, CPT);
-  // the learned step loads through scalar bases (fewer vector instructions where the vector pipe
-  // is the bound); the plain step keeps 64-bit vector addresses, measured faster at 512x256x256
-  bulk_load<L, S, CPT, MLP, MLP>(fin, kind, P, r, in, wdist);
-  if constexpr (MLP) {
-    bulk_finish_mlp<L, S, CPT, REG>(fin, fout, kind, P, r, in, net);
-  } else {
-    // ground rows relax with BGK (wall cells, wgslLbm.ts:186); the branch is wave-uniform
-    if (REG && !(r.iy == 0 && P.ground != 0)) bulk_finish<L, S, CPT, true>(fout, P, r, in);
-    else bulk_finish<L, S, CPT, false>(fout, P, r, in);
-  }
+  bulk_load<L, S, CPT>(fin, kind, P, r, in);
+  // ground rows relax with BGK (wall cells, wgslLbm.ts:186); the branch is wave-uniform
+  if (REG && !(r.iy == 0 && P.ground != 0)) bulk_finish<L, S, CPT, true>(fout, P, r, in);
+  else bulk_finish<L, S, CPT, false>(fout, P, r, in);
+}
+
+// The learned step's bulk block. The network is staged in LDS by the whole block; each wave reads
+// its piece of the network first and issues its population loads before the piece is written and
+// the block synchronises, so the staging's round trip overlaps the populations' instead of preceding
+// it (the LDS write waits only for the loads issued before it: the order is checked in the built code).
+// The population loads use scalar bases (fewer vector instructions where the vector pipe is the
+// bound); the plain step keeps 64-bit vector addresses, measured faster at 512x256x256.
+template <class S, int CPT, bool REG>
+__device__ __forceinline__ void bulk_block_mlp(const typename S::T* __restrict__ fin, typename S::T* __restrict__ fout,
+                                               const uint8_t* __restrict__ kind, const KParams& P, const StepArgs& A, uint32_t b,
+                                               const uint8_t* __restrict__ wdist, const MlpDev* __restrict__ net, MlpDev* snet) {
+  using L = D3Q19;
+  const uint32_t item = bulk_item(A, b, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+  const bool live = item < A.nwaves;  // wave-uniform; every wave takes part in the staging
+  const u32x4_hw piece = stage_net_load(net);
+  // waves past the last item load the last item too (in bounds, results unused): with the loads in
+  // a branch the LDS write after it would wait for every load of the taken side
+  BulkIn<L, S, CPT> in;
+  const RowPos r = item_pos(live ? item : A.nwaves - 1u, A, P, CPT);
+  bulk_load<L, S, CPT, true, true, false>(fin, kind, P, r, in, wdist);
+  stage_net_store(snet, piece);
+  if (!live) return;
+  if constexpr (CPT == 2) bulk_load_fixup<L, S>(P, r, in);  // after the barrier: it waits for the loads
+  bulk_finish_mlp<L, S, CPT, REG>(fin, fout, kind, P, r, in, snet);
 }
 
 // What block b of a step launch does: launch blocks come in groups of 8 (one per XCD), each group
@@ -781,8 +818,7 @@ __global__ __launch_bounds__(BLOCK, step_waves<S>()) void k_step_mlp(const typen
     wall_cell<D3Q19, S>(fin, fout, A, P, (int64_t)br.index * BLOCK + threadIdx.x);
     return;
   }
-  stage_net(net, &snet);  // bulk blocks only: the whole block passes here
-  bulk_block<D3Q19, S, CPT, REG, true>(fin, fout, kind, P, A, br.index, wdist, &snet);
+  bulk_block_mlp<S, CPT, REG>(fin, fout, kind, P, A, br.index, wdist, net, &snet);  // bulk blocks only: the whole block
 }
 
 // One lattice step of many independent domains in one launch (sweeps): the same wall and bulk

@@ -251,13 +251,21 @@ __device__ inline float mlp_nu_of(const MlpDev* __restrict__ net, float z) {
   return net->norm.nu_scale * fminf(softplus(z), MLP_TARGET_CLIP);
 }
 
-// Block-wide copy of the network into LDS (every thread of the block must call it).
-__device__ inline void stage_net(const MlpDev* __restrict__ src, MlpDev* dst) {
-  constexpr int words = (int)(sizeof(MlpDev) / 4);
-  const uint32_t* a = (const uint32_t*)src;
-  uint32_t* b = (uint32_t*)dst;
-  for (int i = threadIdx.x; i < words; i += blockDim.x) b[i] = a[i];
+// Block-wide copy of the network into LDS, in two halves so that a kernel can issue its own global
+// loads between them: stage_net_load reads one 16-byte piece per thread (the network fits in one
+// piece per thread of a 256-thread block), stage_net_store writes it to LDS and synchronises the block
+// (every thread of the block must call both). Global loads complete in issue order, so loads issued
+// between the two halves are not waited for by the LDS write.
+constexpr int MLP_DEV_PIECES = (int)(sizeof(MlpDev) / 16);
+static_assert(sizeof(MlpDev) % 16 == 0 && MLP_DEV_PIECES <= 256, "the network is staged as one 16-byte piece per thread");
+__device__ inline u32x4_hw stage_net_load(const MlpDev* __restrict__ src) {
+  const u32x4_hw* a = (const u32x4_hw*)src;
+  return threadIdx.x < (unsigned)MLP_DEV_PIECES ? a[threadIdx.x] : u32x4_hw{0u, 0u, 0u, 0u};
+}
+__device__ inline void stage_net_store(MlpDev* dst, u32x4_hw v) {
+  if (threadIdx.x < (unsigned)MLP_DEV_PIECES) ((u32x4_hw*)dst)[threadIdx.x] = v;
   __syncthreads();
 }
+__device__ inline void stage_net(const MlpDev* __restrict__ src, MlpDev* dst) { stage_net_store(dst, stage_net_load(src)); }
 
 }  // namespace wt
