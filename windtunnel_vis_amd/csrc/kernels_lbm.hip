@@ -315,6 +315,13 @@ __device__ __forceinline__ RowPos item_pos(uint32_t item, const StepArgs& A, con
   return r;
 }
 
+// The x a lane loads for: its own, or for lanes past the row end (in the last item of a row whose
+// length is not a multiple of the item) the wave's first lane's, so that their loads fall on the
+// lines the wave reads anyway instead of on the row's first lines, which another wave reads.
+__device__ __forceinline__ int load_x(const RowPos& r, int nx, int cpt) {
+  return r.x0 < nx ? r.x0 : r.x0 - (int)(threadIdx.x & 63u) * cpt;
+}
+
 // Row ends of a two-cells-per-lane item loaded through scalar bases (bulk_load with SADDR): the lanes
 // holding a row's first or last pair read their own aligned word for the x-shifted directions and
 // move the needed half into place here (wave-uniform branch: only waves holding a row end).
@@ -323,7 +330,7 @@ __device__ __forceinline__ void bulk_load_fixup(const KParams& P, const RowPos& 
   using Word = typename S::Word;
   constexpr int HALF = 4 * (int)sizeof(Word);
   const int NX = P.nx;
-  const int x0 = r.x0 < NX ? r.x0 : 0;
+  const int x0 = load_x(r, NX, 2);
   const bool yLo = r.iy == 0;
   const bool first = x0 == 0, last = x0 + 2 == NX;
   const int wx = __builtin_amdgcn_readfirstlane(x0);  // the wave's first x (lane 0)
@@ -348,8 +355,8 @@ __device__ __forceinline__ void bulk_load(const typename S::T* __restrict__ fin,
   const int NX = P.nx, NY = P.ny, NZ = P.nz;
   const int64_t n = P.n;
   const int iy = r.iy, iz = r.iz;
-  // lanes past the row end load the row start instead (in bounds, results unused)
-  const int x0 = r.x0 < NX ? r.x0 : 0;
+  // lanes past the row end load what the wave's first lane loads (in bounds, results unused)
+  const int x0 = load_x(r, NX, CPT);
   const int64_t c0 = (int64_t)r.row * NX + x0;
   // every source address below is in bounds whatever the kind, so the loads need not wait for
   // the skip test
