@@ -767,7 +767,7 @@ __device__ __forceinline__ void bulk_block(const typename S::T* __restrict__ fin
 // The learned step's bulk block. The network is staged in LDS by the whole block; each wave reads
 // its piece of the network first and issues its population loads before the piece is written and
 // the block synchronises, so the staging's round trip overlaps the populations' instead of preceding
-// it (the LDS write waits only for the loads issued before it: the order is checked in the built code).
+// it (the piece's load is issued first, so the LDS write's wait leaves the population loads in flight).
 // The population loads use scalar bases (fewer vector instructions where the vector pipe is the
 // bound); the plain step keeps 64-bit vector addresses, measured faster at 512x256x256.
 template <class S, int CPT, bool REG>

@@ -230,9 +230,8 @@ __device__ inline float mlp_group_z(const MlpDev* __restrict__ net, bf16x4_t xb,
   return out[0];
 }
 
-// The network's output for one cell of every lane (x: its packed inputs): groups 2h' and 2h' + 1
-// for cells a and b are evaluated by two calls, cells a first, so that only one cell's operands are
-// live at a time. Returns z + b3 for the lane's own cell.
+// The network's output z + b3 for one cell of every lane (x: the cell's packed inputs). A lane's two
+// cells go through two calls, cell a first, so that only one cell's operands are live at a time.
 __device__ inline float mlp_wave_z(const MlpDev* __restrict__ net, const uint32_t (&x)[4]) {
   const int l = threadIdx.x & 63;
   const auto s0 = __builtin_amdgcn_permlane32_swap(x[0], x[2], false, false);
@@ -254,8 +253,8 @@ __device__ inline float mlp_nu_of(const MlpDev* __restrict__ net, float z) {
 // Block-wide copy of the network into LDS, in two halves so that a kernel can issue its own global
 // loads between them: stage_net_load reads one 16-byte piece per thread (the network fits in one
 // piece per thread of a 256-thread block), stage_net_store writes it to LDS and synchronises the block
-// (every thread of the block must call both). Global loads complete in issue order, so loads issued
-// between the two halves are not waited for by the LDS write.
+// (every thread of the block must call both). The vector-memory wait counter drains in issue order:
+// the LDS write waits for the piece only, and loads issued after it stay in flight.
 constexpr int MLP_DEV_PIECES = (int)(sizeof(MlpDev) / 16);
 static_assert(sizeof(MlpDev) % 16 == 0 && MLP_DEV_PIECES <= 256, "the network is staged as one 16-byte piece per thread");
 __device__ inline u32x4_hw stage_net_load(const MlpDev* __restrict__ src) {
