@@ -506,12 +506,14 @@ __device__ __forceinline__ void bulk_finish(typename S::T* __restrict__ fout, co
 #pragma unroll
       for (int i = 0; i < Q; i++) f[i] = pull_a(i);
       ma = relax<L>(f, (float)x0, gwall, true, P);
+      fold_inlet(ma, P);
     }
     if constexpr (CPT == 2) {
       float f[Q];
 #pragma unroll
       for (int i = 0; i < Q; i++) f[i] = pull_b(i);
       mb = relax<L>(f, (float)(x0 + 1), gwall, true, P);
+      fold_inlet(mb, P);
     }
     store(0, rest_out<L>(ma, true, 0.0f, P), CPT == 2 ? rest_out<L>(mb, true, 0.0f, P) : 0.0f);
 #pragma unroll
@@ -670,6 +672,10 @@ __device__ __forceinline__ void bulk_finish_mlp(const typename S::T* __restrict_
     };
     closure_tau(ma, nu_a);
     if constexpr (CPT == 2) closure_tau(mb, nu_b);
+    if constexpr (reg) {
+      fold_inlet(ma, P);
+      if constexpr (CPT == 2) fold_inlet(mb, P);
+    }
     store(0, rest_out<L>(ma, reg, pull_a(in, 0), P), CPT == 2 ? rest_out<L>(mb, reg, pull_b(in, 0), P) : 0.0f);
 #pragma unroll
     for (int i = 1; i < Q; i++) {

@@ -362,6 +362,7 @@ struct Relax {
   float sIn;           // inlet absorbing-layer weight (0 outside the layer)
   float t0;            // tau before the closure (tau0 + outlet sponge, ground-wall floor)
   float E0, trP;       // E = E0 - k1 trP (regularized) or E0 (BGK)
+  float Ex;            // the regularized even part of the directions with c_x != 0 (= E until fold_inlet)
 };
 
 // The terms of a cell's update that depend only on its x position: the relaxation time before any
@@ -448,6 +449,7 @@ __device__ __forceinline__ Relax relax(const float (&d)[L::Q], float xpos, bool 
     m.k1 = 0.0f;
     m.E = m.E0;
   }
+  m.Ex = m.E;
   return m;
 }
 
@@ -457,6 +459,28 @@ __device__ __forceinline__ void retau(Relax& m, float tau, bool reg) {
   m.om1 = 1.0f - m.omega;
   m.k1 = reg ? 4.5f * m.om1 : 0.0f;
   m.E = reg ? m.E0 - m.k1 * m.trP : m.E0;
+  m.Ex = m.E;
+}
+
+// The inlet absorbing layer folded into a regularized record (wgslLbm.ts:220, :238): the layer's
+// mix out + s (feq_i(1, U) - out) = (1 - s) out + s feq_i(1, U) is linear in the record's terms, and
+// the inlet equilibrium's deviation w_i (3 c_x U + (4.5 c_x^2 - 1.5) U^2) splits into an even part
+// that depends on c_x^2 only and an odd part 3 w_i c_x U. So E, 4.5 rho and k1 scale by 1 - s, the
+// even parts of the c_x = 0 and c_x != 0 directions take s (-1.5 U^2) and s (3 U^2) (E and Ex), and
+// the momentum becomes (1 - s) j + s (U, 0, 0), once per cell instead of a mix per population. With
+// s = 0 (every cell outside the layer) every term keeps its value. Leaves sIn = 0, so the outputs
+// apply no mix of their own; the velocity (c.u) is not scaled.
+__device__ __forceinline__ void fold_inlet(Relax& m, const KParams& P) {
+  const float s = m.sIn, oms = 1.0f - s, U = P.uin;
+  const float E = oms * m.E;
+  m.E = fmaf(s, -1.5f * U * U, E);
+  m.Ex = fmaf(s, 3.0f * U * U, E);
+  m.rho45 *= oms;
+  m.k1 *= oms;
+  m.jx = fmaf(oms, m.jx, s * U);
+  m.jy *= oms;
+  m.jz *= oms;
+  m.sIn = 0.0f;
 }
 
 // Inlet absorbing layer: mix toward feq(1, U), wgslLbm.ts:220, :238.
@@ -473,7 +497,8 @@ __device__ __forceinline__ float rest_out(const Relax& m, bool reg, float d0, co
 }
 
 // Post-collision deviations of the pair (i, opp i), i < opp i:
-// regularized: w_i (E + 4.5 rho cu^2 + k1 Q_i:Pi) +- 3 w_i c_i.j; BGK: omega Deq + (1 - omega) d.
+// regularized: w_i (E + 4.5 rho cu^2 + k1 Q_i:Pi) +- 3 w_i c_i.j (Ex in place of E where c_x != 0);
+// BGK: omega Deq + (1 - omega) d.
 template <class L>
 __device__ __forceinline__ void pair_out(const Relax& m, bool reg, int i, float di, float dop, float& oi, float& oo,
                                          const KParams& P) {
@@ -489,7 +514,7 @@ __device__ __forceinline__ void pair_out(const Relax& m, bool reg, int i, float 
     else if (cx && cz) qp = (m.pxx + m.pzz) + (float)(2 * cx * cz) * m.pxz;
     else if (cy && cz) qp = (m.pyy + m.pzz) + (float)(2 * cy * cz) * m.pyz;
     else qp = cx ? m.pxx : (cy ? m.pyy : m.pzz);
-    const float ev = L::w(i) * (m.E + m.rho45 * cu * cu + m.k1 * qp);
+    const float ev = L::w(i) * ((cx ? m.Ex : m.E) + m.rho45 * cu * cu + m.k1 * qp);
     oi = ev + od;
     oo = ev - od;
   } else {
@@ -523,7 +548,8 @@ template <class L>
 __device__ __forceinline__ void collide(float (&d)[L::Q], float xpos, bool wall, bool gwall, const KParams& P) {
   constexpr int Q = L::Q;
   const bool reg = !(wall || P.collision == 1);
-  const Relax m = relax<L>(d, xpos, gwall, reg, P);
+  Relax m = relax<L>(d, xpos, gwall, reg, P);
+  if (reg) fold_inlet(m, P);
   float out[Q];
   out[0] = rest_out<L>(m, reg, d[0], P);
 #pragma unroll
