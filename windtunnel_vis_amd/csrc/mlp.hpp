@@ -169,13 +169,25 @@ __device__ inline uint32_t relu_bf16x2(uint32_t u) {
   return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(short2_hw, u), (short2_hw){0, 0}));
 }
 
-// Device-side network in the layout the MFMA path reads (prepared once from the FP32 master).
+// Device-side network in the layout the MFMA path reads (prepared once from the FP32 master): the
+// operands of one lane in one 48-byte record and those of one lane half in one 160-byte record, so
+// that two offsets (lane x 48, half x 160) address every read of a group through the instructions'
+// immediate offsets. A 48-byte lane stride keeps the 16-byte reads of 16 consecutive lanes on
+// distinct LDS banks.
+struct alignas(16) MlpLane {
+  bf16x8_t w2b[2];  // A operands of layer 2 per K = 16 chunk
+  bf16x4_t w1a;     // A operand of layer 1
+  uint32_t pad[2];
+};
+struct alignas(16) MlpHalf {
+  float b1d[16];        // layer-1 bias in D layout
+  float b2d[16];        // layer-2 bias in D layout
+  bf16x8_t w3b[2];      // A operand of the output layer per K = 16 chunk (all 32 rows equal)
+};
+static_assert(sizeof(MlpLane) == 48 && sizeof(MlpHalf) == 160, "network record strides");
 struct alignas(16) MlpDev {
-  bf16x4_t w1a[64];     // A operand of layer 1 per lane
-  bf16x8_t w2b[2][64];  // A operands of layer 2 per K = 16 chunk and lane
-  float b1d[2][16];     // layer-1 bias in D layout, per lane half
-  float b2d[2][16];     // layer-2 bias in D layout
-  bf16x8_t w3b[2][2];   // A operand of the output layer per K = 16 chunk and lane half (all 32 rows equal)
+  MlpLane lane[64];
+  MlpHalf half[2];
   float b3;
   float pad[3];
   MlpNorm norm;
@@ -203,30 +215,41 @@ __device__ inline bf16x4_t mlp_operand(uint32_t lo, uint32_t hi) {
   return __builtin_bit_cast(bf16x4_t, (u32x2_hw){lo, hi});
 }
 
-__device__ inline float mlp_group_z(const MlpDev* __restrict__ net, bf16x4_t xb, int g) {
-  // the weights, biases and output weights are read from LDS in each group, each just before its
-  // product: held in registers across the groups (58 of them) they would set the register budget
-  // of the step kernel that fuses this evaluation; the opaque indices keep the compiler from
-  // hoisting the reads out of the groups
-  int l = (int)(threadIdx.x & 63) + 0 * g;
-  asm volatile("" : "+v"(l));
-  const int hg = l >> 5;
-  f32x16_t acc;
+// One group of 32 cell columns through the network. lo = lane x sizeof(MlpLane), ho = lane half x
+// sizeof(MlpHalf): byte offsets of the lane's and the lane half's records. The weights, biases and
+// output weights are read from LDS in each group, each just before its product: held in registers
+// across the groups (58 of them) they would set the register budget of the step kernel that fuses
+// this evaluation; the caller makes the offsets opaque in every group, which keeps the compiler
+// from hoisting the reads out of the groups (and leaves no address arithmetic in them).
+__device__ inline float mlp_group_z(const MlpDev* __restrict__ net, bf16x4_t xb, int lo, int ho) {
+  const MlpLane& ln = *(const MlpLane*)((const char*)net->lane + lo);
+  const MlpHalf& hf = *(const MlpHalf*)((const char*)net->half + ho);
+  // biases as 16-byte reads (the offsets are opaque, so the reads' alignment comes from the type)
+  auto bias = [](const float* b) {
+    typedef float f32x4_hw __attribute__((ext_vector_type(4)));
+    const f32x4_hw* v = (const f32x4_hw*)b;
+    f32x16_t a;
 #pragma unroll
-  for (int r = 0; r < 16; r++) acc[r] = net->b1d[hg][r];
-  acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(net->w1a[l], xb, acc, 0, 0, 0);
+    for (int q = 0; q < 4; q++) {
+      const f32x4_hw t = v[q];
+#pragma unroll
+      for (int r = 0; r < 4; r++) a[4 * q + r] = t[r];
+    }
+    return a;
+  };
+  f32x16_t acc = bias(hf.b1d);
+  acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ln.w1a, xb, acc, 0, 0, 0);
   bf16x8_hw hb0 = relu_pack8(acc, 0), hb1 = relu_pack8(acc, 1);
-#pragma unroll
-  for (int r = 0; r < 16; r++) acc[r] = net->b2d[hg][r];
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w2b[0][l]), hb0, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w2b[1][l]), hb1, acc, 0, 0, 0);
+  acc = bias(hf.b2d);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, ln.w2b[0]), hb0, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, ln.w2b[1]), hb1, acc, 0, 0, 0);
   // output layer on the matrix cores as well: w3 in every row of the A operand, so every register
   // of every lane holds z of the lane's cell column
   hb0 = relu_pack8(acc, 0);
   hb1 = relu_pack8(acc, 1);
   f32x16_t out = {};
-  out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w3b[0][hg]), hb0, out, 0, 0, 0);
-  out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, net->w3b[1][hg]), hb1, out, 0, 0, 0);
+  out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, hf.w3b[0]), hb0, out, 0, 0, 0);
+  out = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_hw, hf.w3b[1]), hb1, out, 0, 0, 0);
   return out[0];
 }
 
@@ -236,11 +259,13 @@ __device__ inline float mlp_wave_z(const MlpDev* __restrict__ net, const uint32_
   const int l = threadIdx.x & 63;
   const auto s0 = __builtin_amdgcn_permlane32_swap(x[0], x[2], false, false);
   const auto s1 = __builtin_amdgcn_permlane32_swap(x[1], x[3], false, false);
+  int lo = l * (int)sizeof(MlpLane), ho = (l >> 5) * (int)sizeof(MlpHalf);
   // one group at a time (not interleaved by the compiler, which would hold two groups' accumulators)
   float z = 0.0f;
 #pragma unroll 1
   for (int g = 0; g < 2; g++) {  // g = 0: the cells of lanes 0-31, g = 1: those of lanes 32-63
-    const float zg = mlp_group_z(net, mlp_operand(g ? s0[1] : s0[0], g ? s1[1] : s1[0]), g);
+    asm volatile("" : "+v"(lo), "+v"(ho));
+    const float zg = mlp_group_z(net, mlp_operand(g ? s0[1] : s0[0], g ? s1[1] : s1[0]), lo, ho);
     if ((l >= 32) == (g == 1)) z = zg;
   }
   return z + net->b3;

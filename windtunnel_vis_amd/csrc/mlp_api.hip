@@ -55,25 +55,25 @@ short host_bf16(float x) {  // round to nearest even, as to_bf16 on the device
   return (short)(u >> 16);
 }
 
-// FP32 master parameters + normalisation -> the MFMA layout of mlp_wave_nu.
+// FP32 master parameters + normalisation -> the MFMA layout that mlp_wave_z reads.
 void build_dev(const wt_mlp* net, MlpDev* d) {
   std::memset(d, 0, sizeof(*d));
   const float* p = net->params;
   for (int l = 0; l < 64; l++) {
     const int row = l % 32, h = l / 32;
-    for (int t = 0; t < 4; t++) d->w1a[l][t] = host_bf16(p[MLP_OFF_W1 + row * MLP_IN + 4 * h + t]);
+    for (int t = 0; t < 4; t++) d->lane[l].w1a[t] = host_bf16(p[MLP_OFF_W1 + row * MLP_IN + 4 * h + t]);
     for (int c = 0; c < 2; c++)
-      for (int j = 0; j < 8; j++) d->w2b[c][l][j] = host_bf16(p[MLP_OFF_W2 + row * MLP_H + mlp_hidden_of(c, h, j)]);
+      for (int j = 0; j < 8; j++) d->lane[l].w2b[c][j] = host_bf16(p[MLP_OFF_W2 + row * MLP_H + mlp_hidden_of(c, h, j)]);
   }
   for (int h = 0; h < 2; h++)
     for (int r = 0; r < 16; r++) {
       const int i = 8 * (r / 4) + 4 * h + (r % 4);
-      d->b1d[h][r] = p[MLP_OFF_B1 + i];
-      d->b2d[h][r] = p[MLP_OFF_B2 + i];
+      d->half[h].b1d[r] = p[MLP_OFF_B1 + i];
+      d->half[h].b2d[r] = p[MLP_OFF_B2 + i];
     }
   for (int c = 0; c < 2; c++)
     for (int h = 0; h < 2; h++)
-      for (int j = 0; j < 8; j++) d->w3b[c][h][j] = host_bf16(p[MLP_OFF_W3 + mlp_hidden_of(c, h, j)]);
+      for (int j = 0; j < 8; j++) d->half[h].w3b[c][j] = host_bf16(p[MLP_OFF_W3 + mlp_hidden_of(c, h, j)]);
   d->b3 = p[MLP_OFF_B3];
   for (int i = 0; i < MLP_IN; i++) {
     const double isig = net->sigma[i] > 0.0f ? 1.0 / net->sigma[i] : 1.0;
