@@ -260,6 +260,10 @@ __device__ inline float mlp_wave_z(const MlpDev* __restrict__ net, const uint32_
   const auto s0 = __builtin_amdgcn_permlane32_swap(x[0], x[2], false, false);
   const auto s1 = __builtin_amdgcn_permlane32_swap(x[1], x[3], false, false);
   int lo = l * (int)sizeof(MlpLane), ho = (l >> 5) * (int)sizeof(MlpHalf);
+  // the wave's issue priority is raised over its dependent matrix chains, so that each chain moves
+  // on as soon as its operands are ready while the other waves' vector work fills the gaps
+  // (0.4 % faster at 256x128x128, five waves per SIMD)
+  __builtin_amdgcn_s_setprio(3);
   // one group at a time (not interleaved by the compiler, which would hold two groups' accumulators)
   float z = 0.0f;
 #pragma unroll 1
@@ -268,6 +272,7 @@ __device__ inline float mlp_wave_z(const MlpDev* __restrict__ net, const uint32_
     const float zg = mlp_group_z(net, mlp_operand(g ? s0[1] : s0[0], g ? s1[1] : s1[0]), lo, ho);
     if ((l >= 32) == (g == 1)) z = zg;
   }
+  __builtin_amdgcn_s_setprio(0);
   return z + net->b3;
 }
 
