@@ -10,6 +10,17 @@ const params = { u: 0.06, nu: 0.01, cs: 0.1, ground: 'moving', spongeNu: 0.04, s
 const flags = new Uint32Array(dims.nx * dims.ny * dims.nz);
 for (let z = 3; z < 7; z++) for (let y = 4; y < 8; y++) for (let x = 9; x < 13; x++) flags[x + dims.nx * (y + dims.ny * z)] = 1;
 
+// resolves once every queued read of `s` has been delivered
+function drain(s) {
+  return new Promise((res, rej) => {
+    const t0 = Date.now();
+    const t = setInterval(() => {
+      if (s.pendingReads.length === 0) { clearInterval(t); res(); }
+      if (Date.now() - t0 > 10000) { clearInterval(t); rej(new Error('reads did not complete')); }
+    }, 2);
+  });
+}
+
 async function main() {
   const out = {};
   // 1. frames: 10 steps, post, flush; the callbacks arrive later, in order
@@ -17,13 +28,6 @@ async function main() {
     const s = new SolverGPU(dims, params);
     s.uploadFlags(flags);
     const got = [];
-    const drain = () => new Promise((res, rej) => {
-      const t0 = Date.now();
-      const t = setInterval(() => {
-        if (s.pendingReads.length === 0) { clearInterval(t); res(); }
-        if (Date.now() - t0 > 10000) { clearInterval(t); rej(new Error('reads did not complete')); }
-      }, 2);
-    });
     // six frames back to back: the first four posts find a free slot; later ones may find all four
     // still waiting for the poll and are then not read back (as in the reference)
     for (let f = 0; f < 6; f++) {
@@ -31,12 +35,12 @@ async function main() {
       s.encodePost(true);
       s.flushReads((st) => got.push(st));
     }
-    await drain();
+    await drain(s);
     out.burst = got.map((st) => st.step);
     // one more frame once the slots are free: its read is always delivered
     s.encodePost(true);
     s.flushReads((st) => got.push(st));
-    await drain();
+    await drain(s);
     out.frames = got.map((st) => ({ step: st.step, fx: st.fx, maxU: st.maxU }));
     out.latest = s.latest.step;
     // the same state sampled synchronously: the last delivered read equals it
@@ -45,7 +49,24 @@ async function main() {
     out.syncFx = sync.fx;
     s.destroy();
   }
-  // 2. calls while a sample is in flight, then destroy with a sample still pending
+  // 2. a read posted in one task and flushed in a later one, after the earlier flush's polls have
+  // run (and found it complete): it goes to the later flush's callback
+  {
+    const s = new SolverGPU(dims, params);
+    s.uploadFlags(flags);
+    const got = [];
+    s.encodeSteps(10);
+    s.encodePost(true);
+    s.flushReads((st) => got.push(['first', st.step]));
+    s.encodeSteps(10);
+    s.encodePost(true);
+    await new Promise((res) => setTimeout(res, 300));
+    s.flushReads((st) => got.push(['second', st.step]));
+    await drain(s);
+    out.late = got;
+    s.destroy();
+  }
+  // 3. calls while a sample is in flight, then destroy with a sample still pending
   {
     const a = new SolverGPU(dims, params);
     const b = new SolverGPU(dims, params);

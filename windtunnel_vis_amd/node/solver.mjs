@@ -107,13 +107,15 @@ export class SolverGPU {
    * them all busy is not read back, as in the reference). Returns at once.
    */
   encodePost(derived = true) {
-    if (native.encodePost(this.h, !!derived)) this.pendingReads.push({ cb: null });
+    if (native.encodePost(this.h, !!derived)) this.pendingReads.push({ cb: null, stats: null });
   }
 
   /**
    * Call after encodePost (SolverGPU.ts:264-287): the reads queued since the last flush deliver
    * their statistics to `onStats` as they complete, in any later turn of the event loop, and
-   * `latest` keeps the newest by step. Never blocks: completed reads are polled.
+   * `latest` keeps the newest by step. Never blocks: completed reads are polled. A read that
+   * completes before its flush (posted after the previous one) is held until that flush, as the
+   * reference maps a read only when it is flushed.
    */
   flushReads(onStats) {
     for (const r of this.pendingReads) if (!r.cb) r.cb = onStats || (() => {});
@@ -125,11 +127,18 @@ export class SolverGPU {
     const poll = () => {
       this.pollTimer = null;
       if (!this.h) return;
+      // reads complete in posting order: hand the completed ones to the oldest open entries
       const { stats } = native.flushReads(this.h);
-      for (const s of stats) {
-        const r = this.pendingReads.shift();  // reads complete in posting order
-        if (!this.latest || s.step >= this.latest.step) this.latest = s;
-        if (r && r.cb) r.cb(s);
+      let k = 0;
+      for (const r of this.pendingReads) {
+        if (k >= stats.length) break;
+        if (!r.stats) r.stats = stats[k++];
+      }
+      // deliver the completed, flushed prefix (flushed entries precede the ones posted after)
+      while (this.pendingReads.length && this.pendingReads[0].stats && this.pendingReads[0].cb) {
+        const r = this.pendingReads.shift();
+        if (!this.latest || r.stats.step >= this.latest.step) this.latest = r.stats;
+        r.cb(r.stats);
       }
       if (this.pendingReads.some((r) => r.cb)) this.pollTimer = setTimeout(poll, 1);
     };
