@@ -287,8 +287,9 @@ struct BulkIn {
 
 // Row coordinates of an item (wave-uniform).
 struct RowPos {
-  uint32_t row;
+  uint32_t row;         // wave-uniform
   int iy, iz, x0;
+  int xs;               // the first x of the item's chunk
   const int64_t* pull;  // the pull table (source offsets per row class)
 };
 
@@ -304,22 +305,23 @@ __device__ __forceinline__ T* sgpr_base(T* p) {
 }
 
 __device__ __forceinline__ RowPos item_pos(uint32_t item, const StepArgs& A, const KParams& P, int cpt) {
+  const ItemCell c = item_cell(A, item, threadIdx.x & 63u, cpt);
   RowPos r;
   r.pull = A.pull;
-  r.row = A.dwpr.div(item);
-  const uint32_t chunk = item - r.row * A.wpr;
+  r.row = c.row;
   const uint32_t z = P.divy.div(r.row);
   r.iz = (int)z;
   r.iy = (int)(r.row - z * (uint32_t)P.ny);
-  r.x0 = (int)(chunk * 64u + (threadIdx.x & 63u)) * cpt;
+  r.x0 = c.x0;
+  r.xs = c.xs;
   return r;
 }
 
-// The x a lane loads for: its own, or for lanes past the row end (in the last item of a row whose
-// length is not a multiple of the item) the wave's first lane's, so that their loads fall on the
-// lines the wave reads anyway instead of on the row's first lines, which another wave reads.
-__device__ __forceinline__ int load_x(const RowPos& r, int nx, int cpt) {
-  return r.x0 < nx ? r.x0 : r.x0 - (int)(threadIdx.x & 63u) * cpt;
+// The x a lane loads for: its own, or for lanes past the row end (in the last chunk of a row whose
+// length is not a multiple of the chunk) the chunk's first, so that their loads fall on the lines
+// the wave reads anyway instead of on the row's first lines, which another wave reads.
+__device__ __forceinline__ int load_x(const RowPos& r, int nx, int) {
+  return r.x0 < nx ? r.x0 : r.xs;
 }
 
 // Row ends of a two-cells-per-lane item loaded through scalar bases (bulk_load with SADDR): the lanes
@@ -1174,7 +1176,7 @@ std::vector<uint32_t> build_roles(const StepArgs& A, const std::vector<uint32_t>
   std::vector<std::vector<uint32_t>> after(nbg + 1);
   for (uint32_t k = 0; k < nwg; k++) {
     const int64_t row = k < wall_rows.size() ? std::min<int64_t>((int64_t)wall_rows[k] + ny + 1, rows - 1) : rows - 1;
-    const int64_t item = std::min<int64_t>(row * A.wpr + A.wpr - 1, (int64_t)A.nwaves - 1);
+    const int64_t item = std::min<int64_t>(last_item_of_row(A, row), (int64_t)A.nwaves - 1);
     const int64_t run = item / A.xk;  // runs of xk items, dealt round-robin to the XCDs (bulk_block)
     const int64_t lw = (run / 8) * A.xk + item % A.xk;  // wave index on its XCD
     after[std::min<int64_t>(lw / 4, nbg)].push_back(k);  // bulk group = (bulk block) / 8 = lw / 4
@@ -1199,14 +1201,12 @@ StepArgs step_args(const LaunchCtx& c, const uint32_t* list, const uint32_t* met
   A.wcap = cap;
   A.nwall = count;
   A.nwb = (uint32_t)(((count + BLOCK - 1) / BLOCK + 7) / 8 * 8);  // a multiple of 8 keeps bulk blocks XCD-aligned
-  A.wpr = (uint32_t)((c.P.nx + 64 * cpt - 1) / (64 * cpt));
-  A.dwpr.init(A.wpr);
-  A.nwaves = (uint32_t)((int64_t)c.P.ny * c.P.nz * A.wpr);
+  set_item_layout(A, c.P.nx, c.P.ny, c.P.nz, cpt);
+  A.pad = 0;
   // chunk <= 0: one contiguous eighth of the items per XCD
   const int64_t per = ((int64_t)A.nwaves + 7) / 8;
   A.xk = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(chunk > 0 ? chunk : per, per));
   A.dxk.init(A.xk);
-  A.pad = 0;
   const int64_t runs = ((int64_t)A.nwaves + A.xk - 1) / A.xk;
   const int64_t waves_per_xcd = (runs + 7) / 8 * (int64_t)A.xk;
   A.nbb = (uint32_t)(8 * ((waves_per_xcd + 3) / 4));

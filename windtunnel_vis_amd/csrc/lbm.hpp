@@ -192,8 +192,8 @@ struct FastDiv {  // unsigned 32-bit division by an invariant d >= 1 (round-up m
     m = (uint32_t)((((1ull << l) - div) << 32) / div + 1);
     s = l;
   }
-  __device__ uint32_t div(uint32_t x) const {
-    uint32_t t = __umulhi(m, x);
+  __host__ __device__ uint32_t div(uint32_t x) const {
+    const uint32_t t = (uint32_t)(((uint64_t)m * x) >> 32);
     return s == 0 ? x : (t + ((x - t) >> 1)) >> (s - 1);
   }
 };
@@ -270,7 +270,7 @@ struct StepArgs {
   int64_t wcap;
   uint32_t nwb;     // wall blocks
   uint32_t wpr;     // waves per x-row
-  uint32_t nwaves;  // bulk items (one wave-sized chunk of an x-row each)
+  uint32_t nwaves;  // bulk items (one wave-sized chunk of an x-row each, see item_cell)
   uint32_t nbb;     // bulk blocks
   uint32_t xk;      // items per XCD chunk (consecutive items run on one XCD)
   uint32_t pad;
@@ -281,6 +281,33 @@ struct StepArgs {
   // 8g..8g+7); (nwb + nbb) / 8 entries
   const uint32_t* role;
 };
+
+// The bulk items of a step: items walk the rows in order (x fastest, then y, then z), one wave-sized
+// chunk (64 cpt cells) of a row each. Shared by the kernels (item_pos), the host's role table
+// (build_roles) and the host check of the layout (tests/native/item_layout_check.hip).
+__host__ inline void set_item_layout(StepArgs& A, int64_t nx, int64_t ny, int64_t nz, int cpt) {
+  A.wpr = (uint32_t)((nx + 64 * cpt - 1) / (64 * cpt));
+  A.dwpr.init(A.wpr);
+  A.nwaves = (uint32_t)(ny * nz * A.wpr);
+}
+
+// Where lane `lane` of bulk item `item` works: its row (wave-uniform), x0 (the lane's first cell)
+// and xs (the first x of the item's chunk, which lanes past the row end load instead of their own).
+struct ItemCell {
+  uint32_t row;
+  int x0, xs;
+};
+__host__ __device__ inline ItemCell item_cell(const StepArgs& A, uint32_t item, uint32_t lane, int cpt) {
+  ItemCell c;
+  c.row = A.dwpr.div(item);
+  const uint32_t chunk = item - c.row * A.wpr;
+  c.xs = (int)(chunk * 64u) * cpt;
+  c.x0 = c.xs + (int)lane * cpt;
+  return c;
+}
+
+// The last item of a row (the one holding its last chunk).
+__host__ inline int64_t last_item_of_row(const StepArgs& A, int64_t row) { return row * A.wpr + A.wpr - 1; }
 
 // One domain of a batched step (k_step_batch): its parameters, launch layout, buffers and where its
 // blocks sit in the batched grid (each domain's nwb + nbb blocks in a contiguous run, ordered by its
