@@ -361,7 +361,7 @@ def closure_bench():
 
 def cpu_threads():
     """Threads for the CPU baseline: every CPU this process may run on, capped by OMP_NUM_THREADS
-    when that is set (the GPU box gives one GPU's job a 16-core share and sets it to 16)."""
+    when that is set (a host that gives each GPU's job a share of its cores sets it to that share)."""
     try:
         avail = len(os.sched_getaffinity(0))
     except (AttributeError, OSError):
