@@ -402,6 +402,14 @@ def roofline_block(r, copy_gbs, dims, traffic_name):
             "measured_copy_gbs": round(copy_gbs, 1)}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One progress line per finished part on stderr (stdout carries only the result line)."""
+    print("[bench %6.1f s] %s" % (time.perf_counter() - _T0, what), file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,7 +444,9 @@ def main():
         args.no_small = not args.learned_step
     copy_mib = args.copy_mib or (256 if args.profile_run else 2048)
     copy_gbs = copy_probe(copy_mib << 20, 3 if args.profile_run else 5)
+    progress("copy probe %.0f GB/s" % copy_gbs)
     r = run_gpu(dims, args.steps, args.warmup, args.precision, chunk=args.chunk)
+    progress("%dx%dx%d step: %.1f MLUPS" % (dims + (r["mlups"],)))
     out = {
         "metric": METRIC, "value": round(r["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 5), "higher_is_better": True,
@@ -463,6 +473,7 @@ def main():
                                "sample": "%d steps of the same %dx%dx%d sedan domain (the GPU run's flags, wall distances and "
                                          "wheels: Bouzidi links, rolling wheels, belt), FP32 CPU restatement with FP16 store "
                                          "rounding, OpenMP on %d threads, %.1f s" % ((steps,) + dims + (threads, cb["seconds"]))}
+        progress("CPU baseline %.1f MLUPS on %d threads" % (cb["value"], threads))
     r["solver"].destroy()
     if not args.no_small and dims != small:
         rs = run_gpu(small, args.steps, args.warmup, args.precision, chunk=args.chunk)
@@ -483,6 +494,7 @@ def main():
                                          "on, run until the convergence rule holds; *_second_half_5k: steps 2.5k-5k "
                                          "of the configuration's 5k-step run" % ph["L"]}
         out["d3q19_%s_256x128x128" % args.precision] = line
+        progress("256x128x128 step and physics")
         plain_ms = rs["k_avg_ms"]
         rs["solver"].destroy()
     elif dims == small:
@@ -490,6 +502,7 @@ def main():
     if (args.learned_step or not (args.no_learned_step or args.profile_run)) and (dims == small or not args.no_small):
         out["learned_step"] = learned_step_bench(small, max(min(args.steps, 2000) // 4, 50), args.warmup, plain_ms,
                                                  args.precision)
+        progress("learned step")
     if not args.no_physics:
         from windtunnel_vis_amd.validation import cylinder_512x128
         from windtunnel_vis_amd import SolverGPU
@@ -498,14 +511,19 @@ def main():
                         "cycles": w["cycles"], "steps": w["steps"]}
         out["cylinder_512x128"] = {"window_20k": rd(c1["window_20k"]), "developed": rd(c1["developed"]),
                                    "unstable": c1["unstable"], "seconds": round(c1["seconds"], 2)}
+        progress("cylinder 512x128")
     if not args.no_large:
         out["shard_128x256x256"] = shard_bench(max(min(args.steps, 2000) // 4, 100), args.warmup, args.precision)
+        progress("x-slab shard")
     if not args.no_sweep:
         out["batched_sweep"] = sweep_bench()
+        progress("batched sweep")
     if not args.no_closure:
         out["learned_closure"] = closure_bench()
+        progress("learned-closure study")
     if not args.no_stress:
         out["precision_stress_1024x512x512"] = precision_stress()
+        progress("precision stress")
     print(json.dumps(out))
 
 
