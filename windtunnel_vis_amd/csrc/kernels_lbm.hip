@@ -980,9 +980,20 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
     vel[idx] = make_ushort4((unsigned short)(lo & 0xffffu), (unsigned short)(lo >> 16), (unsigned short)(hi & 0xffffu),
                             (unsigned short)(hi >> 16));
   }
-  // block reduction: waves via cross-lane shuffles, then LDS across the 4 waves
-  F0 = wave_sum(F0); F1 = wave_sum(F1); F2 = wave_sum(F2);
-  B0 = wave_sum(B0); B1 = wave_sum(B1); B2 = wave_sum(B2); B3 = wave_sum(B3);
+  // block reduction: waves via cross-lane shuffles, then LDS across the 4 waves. The counts
+  // (reference-column cells, bad cells, solid cells) are popcounts of the wave's ballot, exact; the
+  // sums (forces on the body's links, density over the reference column) are reduced only in waves
+  // where some lane holds a non-zero term: a wave of zero terms sums to zero either way, and the
+  // other waves add in the same order as before
+  B1 = (double)__popcll(__ballot(B1 != 0.0));
+  B2 = (double)__popcll(__ballot(B2 != 0.0));
+  B3 = (double)__popcll(__ballot(B3 != 0.0));
+  if (__any(F0 != 0.0 || F1 != 0.0 || F2 != 0.0)) {
+    F0 = wave_sum(F0); F1 = wave_sum(F1); F2 = wave_sum(F2);
+  } else {
+    F0 = F1 = F2 = 0.0;
+  }
+  B0 = __any(B0 != 0.0) ? wave_sum(B0) : 0.0;
   amax = wave_max(amax);
   __shared__ double sh[BLOCK / 64][8];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -999,14 +1010,17 @@ __global__ __launch_bounds__(BLOCK) void k_macro(const typename S::T* __restrict
   }
 }
 
-// Final reduction of the block partials (wgslLbm.ts:491-521) into result[8], FP64.
-__global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ partials, int64_t count, double* __restrict__ result) {
+// Fixed-order reduction of `count` records of 8 doubles (sums; slot 3 a maximum) into out[8] by one
+// block of T threads: thread t folds records t, t + T, ... in order, then each wave (cross-lane, a
+// fixed tree) and the block (LDS, in wave order). The result depends only on count.
+template <int T>
+__device__ __forceinline__ void reduce_records(const double* __restrict__ p, int64_t count, double* __restrict__ out) {
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t b = threadIdx.x; b < count; b += 1024) {
+  for (int64_t b = threadIdx.x; b < count; b += T) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) acc[k] = (k == 3) ? fmax(acc[k], partials[b * 8 + k]) : acc[k] + partials[b * 8 + k];
+    for (int k = 0; k < 8; k++) acc[k] = (k == 3) ? fmax(acc[k], p[b * 8 + k]) : acc[k] + p[b * 8 + k];
   }
-  __shared__ double sh[16][8];
+  __shared__ double sh[T / 64][8];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
@@ -1021,9 +1035,26 @@ __global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ part
   if (threadIdx.x < 8) {
     const int k = threadIdx.x;
     double v = sh[0][k];
-    for (int w = 1; w < 16; w++) v = (k == 3) ? fmax(v, sh[w][k]) : v + sh[w][k];
-    result[k] = v;
+    for (int w = 1; w < T / 64; w++) v = (k == 3) ? fmax(v, sh[w][k]) : v + sh[w][k];
+    out[k] = v;
   }
+}
+
+// The block partials are reduced in two stages (wgslLbm.ts:491-521 reduces them in one workgroup):
+// chunks of REDUCE_CHUNK records by one 256-thread block each, then the chunk results by one block.
+// A single block reading all of them (8 MB at 512x256x256) is bound by one CU's bandwidth (196 us).
+constexpr int64_t REDUCE_CHUNK = 512;
+__host__ __device__ constexpr int64_t reduce_chunks(int64_t records) { return (records + REDUCE_CHUNK - 1) / REDUCE_CHUNK; }
+
+__global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict__ partials, int64_t count, double* __restrict__ stage) {
+  const int64_t first = (int64_t)blockIdx.x * REDUCE_CHUNK;
+  const int64_t m = count - first < REDUCE_CHUNK ? count - first : REDUCE_CHUNK;
+  reduce_records<256>(partials + first * 8, m, stage + (int64_t)blockIdx.x * 8);
+}
+
+// Final reduction into result[8], FP64.
+__global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ partials, int64_t count, double* __restrict__ result) {
+  reduce_records<1024>(partials, count, result);
 }
 
 // Convert between device storage and full-value host floats. The host side is direction-major,
@@ -1271,7 +1302,10 @@ void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs
   });
 }
 
-int64_t macro_blocks(int64_t n) { return (int64_t)blocks_for(n); }
+int64_t macro_records(int64_t n) {
+  const int64_t blocks = (int64_t)blocks_for(n);
+  return blocks + reduce_chunks(blocks);
+}
 
 void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, float* meanB, double* partials, double* result) {
   dispatch(c.P.q, c.prec, [&](auto l, auto s) {
@@ -1279,7 +1313,11 @@ void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, f
     hipLaunchKernelGGL((k_macro<L, S>), dim3(blocks_for(c.P.n)), dim3(BLOCK), 0, c.stream, (const typename S::T*)fin, c.flags,
                        c.sdf, c.kind, (ushort4*)vel, (float4*)meanA, meanB, partials, c.P, *c.W);
   });
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c.stream, partials, (int64_t)blocks_for(c.P.n), result);
+  // the block partials, then the chunk results right after them in the same buffer (macro_records)
+  const int64_t blocks = (int64_t)blocks_for(c.P.n), chunks = reduce_chunks(blocks);
+  double* stage = partials + blocks * 8;
+  hipLaunchKernelGGL(k_reduce_chunks, dim3((uint32_t)chunks), dim3(256), 0, c.stream, (const double*)partials, blocks, stage);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c.stream, (const double*)stage, chunks, result);
 }
 
 void launch_unpack(const LaunchCtx& c, const void* f, float* out) {

@@ -44,7 +44,7 @@ void launch_step(const LaunchCtx& c, const void* fin, void* fout, const StepArgs
 void launch_step_batch(hipStream_t st, int q, int prec, int cpt, bool reg, const DomainDesc* descs, const uint32_t* dmap,
                        uint32_t blocks, int parity);
 int cells_per_thread(const KParams& P);
-int64_t macro_blocks(int64_t n);
+int64_t macro_records(int64_t n);  // 8-double records of the macro pass's partials buffer
 void launch_macro(const LaunchCtx& c, const void* fin, void* vel, void* meanA, float* meanB, double* partials, double* result);
 void launch_unpack(const LaunchCtx& c, const void* f, float* out);
 void launch_unpack_planes(const LaunchCtx& c, const void* f, int32_t z0, int32_t count, float* out);

@@ -364,7 +364,7 @@ int wt_create(const wt_dims* dims, const wt_params* params, wt_solver** out) {
   derive(s);
   const int64_t sbytes = s->prec == WT_PREC_F32 ? 4 : 2;
   const int64_t fbytes = (int64_t)s->q * n * sbytes;
-  const int64_t nblocks = macro_blocks(n);
+  const int64_t nrecords = macro_records(n);
   auto alloc = [&](void** p, int64_t bytes) -> bool {
     if (hipMalloc(p, (size_t)std::max<int64_t>(bytes, 256)) != hipSuccess) return false;
     s->bytes += bytes;
@@ -374,7 +374,7 @@ int wt_create(const wt_dims* dims, const wt_params* params, wt_solver** out) {
   bool ok = alloc(&s->f[0], fbytes + 64) && alloc(&s->f[1], fbytes + 64) && alloc((void**)&s->flags, 4 * n) &&
             alloc((void**)&s->sdf, 4 * n) && alloc((void**)&s->kind, n + 64) && alloc((void**)&s->wlist, 4 * (n + 1)) && alloc((void**)&s->wmask, 4 * (n + 1)) &&
             alloc((void**)&s->counts, 64) && alloc((void**)&s->vel, 8 * n) && alloc((void**)&s->meanA, 16 * n) &&
-            alloc((void**)&s->meanB, 4 * n) && alloc((void**)&s->partials, 64 * nblocks) && alloc((void**)&s->result, 64);
+            alloc((void**)&s->meanB, 4 * n) && alloc((void**)&s->partials, 64 * nrecords) && alloc((void**)&s->result, 64);
   if (!ok) {
     wt_destroy(s);
     return fail(WT_E_NOMEM, "device allocation failed");
