@@ -39,7 +39,8 @@ __global__ __launch_bounds__(FBLOCK) void k_derived(const uint2* __restrict__ ve
   const int iy = (int)(row - zz * (uint32_t)P.ny), iz = (int)zz;
   const int NX = P.nx, NY = P.ny, NZ = P.nz;
   const int64_t sx = 1, sy = NX, sz = (int64_t)NX * NY;
-  const bool solid = (kind[idx] & KIND_SOLID) != 0;
+  const uint8_t kd = kind[idx];
+  const bool solid = (kd & KIND_SOLID) != 0;
   float4 vo = make_float4(0.f, 0.f, 0.f, 0.f);
   if (!solid) {
     // central differences, one-sided (halved) at a clamped face
@@ -67,19 +68,9 @@ __global__ __launch_bounds__(FBLOCK) void k_derived(const uint2* __restrict__ ve
         S2 += s * s;
         W2 += w * w;
       }
-    // Q fades out within two cells of a solid along the axes (1 cell: off, 2 cells: half)
-    auto solid_at = [&](int x, int y, int z) -> bool {
-      if (x < 0 || y < 0 || z < 0 || x >= NX || y >= NY || z >= NZ) return false;
-      return (kind[(int64_t)x + sy * y + sz * z] & KIND_SOLID) != 0;
-    };
-    float near = 0.f;
-#pragma unroll
-    for (int d = 1; d <= 2; d++) {
-      const float v = d == 1 ? 1.0f : 0.5f;
-      bool hit = solid_at(ix + d, iy, iz) || solid_at(ix - d, iy, iz) || solid_at(ix, iy + d, iz) || solid_at(ix, iy - d, iz);
-      if (DIM == 3) hit = hit || solid_at(ix, iy, iz + d) || solid_at(ix, iy, iz - d);
-      if (hit) near = fmaxf(near, v);
-    }
+    // Q fades out within two cells of a solid along the axes (1 cell: off, 2 cells: half); the
+    // classification pass marks those cells (KIND_NEAR1 / KIND_NEAR2) once per geometry
+    const float near = (kd & KIND_NEAR1) ? 1.0f : (kd & KIND_NEAR2) ? 0.5f : 0.0f;
     vo = make_float4(wx, wy, wz, 0.5f * (W2 - S2) * (1.0f - near));
   }
   const float4 m = meanA[idx];

@@ -83,6 +83,19 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const uint32_t* __restrict__
         }
     if (hold) k |= KIND_BELT_HOLD;
   }
+  // the derived pass's near-wall fade of Q depends on the geometry only: solid cells along the axes
+  // at one and two cells (outside the domain counts as fluid), decided here once per geometry
+  auto solid_at = [&](int x, int y, int z) -> bool {
+    if (x < 0 || y < 0 || z < 0 || x >= P.nx || y >= P.ny || z >= P.nz) return false;
+    return flags[(int64_t)x + (int64_t)P.nx * ((int64_t)y + (int64_t)P.ny * z)] != 0u;
+  };
+  auto axis_hit = [&](int d) {
+    bool hit = solid_at(ix + d, iy, iz) || solid_at(ix - d, iy, iz) || solid_at(ix, iy + d, iz) || solid_at(ix, iy - d, iz);
+    if (L::DIM == 3) hit = hit || solid_at(ix, iy, iz + d) || solid_at(ix, iy, iz - d);
+    return hit;
+  };
+  if (axis_hit(1)) k |= KIND_NEAR1;
+  else if (axis_hit(2)) k |= KIND_NEAR2;
   kind[idx] = k;
 }
 

@@ -41,6 +41,8 @@ enum : uint8_t {
   KIND_SOLID = 1,      // solid cell (rest state)
   KIND_WALL = 2,       // fluid cell with at least one plain pull from a solid cell: wall kernel
   KIND_BELT_HOLD = 4,  // floor cell where a tyre contact patch holds the belt still (wgslLbm.ts:276-287)
+  KIND_NEAR1 = 8,      // fluid cell with a solid cell one cell away along an axis (the Q-criterion fade)
+  KIND_NEAR2 = 16,     // fluid cell with a solid cell two cells away along an axis and none at one
 };
 
 // ---------------------------------------------------------------------------------------------
