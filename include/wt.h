@@ -215,6 +215,15 @@ int wt_flush_reads(wt_solver* s, wt_stats* out, int32_t cap, int32_t* count, int
 int wt_sample_raw(wt_solver* s, double* out8);
 /* Block until the handle's stream is idle. */
 int wt_sync(wt_solver* s);
+/* Queue fences, for the reference's frame throttle (backend/webgpu.ts:92-94, :120: frames in flight
+ * counted with device.queue.onSubmittedWorkDone()). The handle's stream is its queue: wt_fence
+ * marks the point after all work queued on the handle so far and returns its ticket (> 0);
+ * wt_fence_done sets *done to 1 once the GPU has passed that point (non-blocking; it may report 0
+ * a little longer for a ticket whose record was reused after 16 newer fences); wt_fence_wait
+ * blocks until it has. Tickets from another handle, or not yet issued, are WT_E_ARG. */
+int wt_fence(wt_solver* s, int64_t* ticket);
+int wt_fence_done(wt_solver* s, int64_t ticket, int32_t* done);
+int wt_fence_wait(wt_solver* s, int64_t ticket);
 
 /* ---- readback / upload ------------------------------------------------------------------ */
 

@@ -66,7 +66,39 @@ async function main() {
     out.late = got;
     s.destroy();
   }
-  // 3. calls while a sample is in flight, then destroy with a sample still pending
+  // 3. the reference's throttled frame loop (backend/webgpu.ts:92-131): a frame is submitted only
+  // while fewer than two are in flight, counted with onSubmittedWorkDone, and setParams ramps the
+  // speed every frame (app.ts:792-810); setParams does not wait for the queue
+  {
+    const s = new SolverGPU(dims, params);
+    s.uploadFlags(flags);
+    let inFlight = 0, maxInFlight = 0, frames = 0;
+    const got = [];
+    await new Promise((resolve, reject) => {
+      const t0 = Date.now();
+      const tick = () => {
+        if (frames >= 40) {
+          if (inFlight === 0) { resolve(); return; }
+        } else if (inFlight < 2) {
+          s.setParams({ u: 0.06 * (0.5 + 0.5 * frames / 40) });
+          s.encodeSteps(5);
+          s.encodePost(true);
+          inFlight++;
+          frames++;
+          maxInFlight = Math.max(maxInFlight, inFlight);
+          s.flushReads((st) => got.push(st.step));
+          s.onSubmittedWorkDone().then(() => { inFlight--; });
+        }
+        if (Date.now() - t0 > 20000) { reject(new Error('frames did not complete')); return; }
+        setTimeout(tick, 0);
+      };
+      tick();
+    });
+    await drain(s);
+    out.throttled = { frames, maxInFlight, reads: got, step: s.stepCount };
+    s.destroy();
+  }
+  // 4. calls while a sample is in flight, then destroy with a sample still pending
   {
     const a = new SolverGPU(dims, params);
     const b = new SolverGPU(dims, params);

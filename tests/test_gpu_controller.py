@@ -84,3 +84,33 @@ def test_controller_runs_sphere():
     assert c.cur["nu"] == pytest.approx(c.flow["nu"], rel=1e-3)  # start-up damping has decayed
     assert math.isfinite(r["cd"]) and 0.2 < r["cd"] < 3.0 and r["maxMach"] < 0.6
     s.destroy()
+
+
+def test_fences_follow_the_queue():
+    """wt_fence / wt_fence_done / wt_fence_wait: a fence passes once the work queued before it is
+    done (waited for, or after a sync); tickets count up; the ring of 16 records is reused without
+    losing older tickets; a ticket not yet issued is refused. setParams with only a speed change
+    leaves the queue running (the fence after it still follows the queued steps)."""
+    from windtunnel_vis_amd import SolverGPU
+    s = SolverGPU((64, 32, 24), dict(u=0.05, nu=0.02, cs=0.1, ground="moving", precision="f16"))
+    try:
+        s.step(50)
+        s.setParams(dict(u=0.06))
+        t1 = s.fence()
+        s.fenceWait(t1)
+        assert s.fenceDone(t1)
+        ts = []
+        for k in range(40):
+            s.step(2)
+            ts.append(s.fence())
+        assert ts == list(range(t1 + 1, t1 + 41))
+        s.fenceWait(ts[-1])
+        assert all(s.fenceDone(t) for t in ts)  # older tickets, their records reused, are passed too
+        s.step(10)
+        t2 = s.fence()
+        s.sync()
+        assert s.fenceDone(t2)
+        with pytest.raises(RuntimeError):
+            s.fenceDone(t2 + 1)
+    finally:
+        s.destroy()

@@ -60,7 +60,9 @@ def test_node_async_frames_and_calls_during_a_sample():
     """encodePost / flushReads (the reference's out-of-order-tolerant frame reads): every delivered
     read carries its post's step, in order, and the last equals a synchronous sample of the same
     state; a read posted after a flush and completed before the next one goes to the next one's
-    callback. encodeSteps during an un-awaited stepAndSample runs after it (the handle lock): the results
+    callback; the reference's throttled frame loop (at most two frames in flight, counted with
+    onSubmittedWorkDone, setParams every frame) keeps its bound and delivers its reads. encodeSteps
+    during an un-awaited stepAndSample runs after it (the handle lock): the results
     equal a strictly sequential run on a second solver. destroy() during a sample is deferred: the
     sample still resolves and later calls throw."""
     r = subprocess.run([NODE, "tests/node/async_frames.mjs"], cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
@@ -75,6 +77,9 @@ def test_node_async_frames_and_calls_during_a_sample():
     assert o["latest"] == 60 and o["syncStep"] == 60
     assert o["frames"][-1]["fx"] == o["syncFx"]
     assert o["late"] == [["first", 10], ["second", 20]]
+    th = o["throttled"]
+    assert th["frames"] == 40 and th["step"] == 200 and 1 <= th["maxInFlight"] <= 2
+    assert th["reads"] == sorted(th["reads"]) and len(th["reads"]) >= 36 and th["reads"][-1] == 200
     race = o["race"]
     assert race["a1"] == race["b1"] and race["a2"] == race["b2"] and race["a2"][0] == 230
     assert o["destroyed"] == {"step": 300, "unstable": False, "threwAfterDestroy": True}

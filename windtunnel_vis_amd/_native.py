@@ -131,6 +131,9 @@ SIGNATURES = {
     "wt_encode_post": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "wt_flush_reads": (C.c_int, [_P, C.POINTER(Stats), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "wt_sync": (C.c_int, [_P]),
+    "wt_fence": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "wt_fence_done": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_int32)]),
+    "wt_fence_wait": (C.c_int, [_P, C.c_int64]),
     "wt_read_populations": (C.c_int, [_P, _P]),
     "wt_upload_populations": (C.c_int, [_P, _P]),
     "wt_read_population_planes": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),

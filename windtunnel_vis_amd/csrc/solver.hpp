@@ -96,6 +96,12 @@ struct wt_solver {
   // when every slot is in flight; `read_mu` guards it, since wt_sample_wait may run on another thread
   std::vector<ReadSlot*> samples;
   std::mutex read_mu;
+  // queue fences (wt_fence): a ring of events; fence_ticket[k] is the ticket last recorded in slot
+  // k, fence_done the newest ticket known to be passed (in stream order every older one is too)
+  static constexpr int FENCES = 16;
+  hipEvent_t fence_ev[FENCES] = {};
+  int64_t fence_ticket[FENCES] = {};
+  int64_t fence_seq = 0, fence_done = 0;
 
   wt::LaunchCtx ctx() const {
     wt::LaunchCtx c;

@@ -84,10 +84,21 @@ int run_init(wt_solver* s) {
 int prep_walls(wt_solver* s) {
   const size_t len = (size_t)pull_table_len(s->q);
   if (!s->pull) HIP_TRY(hipMalloc((void**)&s->pull, len * sizeof(int64_t)));
-  s->h_pull.assign(len, 0);
-  pull_table(s->P, s->h_pull.data());
-  HIP_TRY(hipMemcpyAsync(s->pull, s->h_pull.data(), len * sizeof(int64_t), hipMemcpyHostToDevice, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  // The pull table depends on the grid and the boundary modes only: the per-frame parameter ramps
+  // (speed, viscosity, averaging rate) leave it unchanged, and then nothing is uploaded and the
+  // call does not wait for the stream. A changed table is uploaded in stream order (after the steps
+  // already queued, which read the old one) and waited for, since the host copy is its source.
+  std::vector<int64_t> t(len, 0);
+  pull_table(s->P, t.data());
+  if (t != s->h_pull) {
+    hipError_t e = hipMemcpyAsync(s->pull, t.data(), len * sizeof(int64_t), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+      s->h_pull.clear();  // the device table is not known to match any host copy
+      return fail(WT_E_HIP, std::string("pull table upload: ") + hipGetErrorString(e));
+    }
+    s->h_pull.swap(t);
+  }
   if (!s->wlink) return WT_OK;
   LaunchCtx c = s->ctx();
   launch_wallprep(c, s->wlist, s->wmask, s->wlink, s->wmeta, s->nwall, s->wcap);
@@ -410,6 +421,8 @@ int wt_destroy(wt_solver* s) {
   if (!s) return WT_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto e : s->ev) (void)hipEventDestroy(e);
+  for (auto e : s->fence_ev)
+    if (e) (void)hipEventDestroy(e);
   void* ptrs[] = {s->f[0], s->f[1], s->flags, s->sdf, s->kind, s->wlist, s->wmask, s->wlink, s->wmeta, s->wdist, s->mlp, s->counts, s->vel, s->meanA, s->meanB, s->partials, s->result, s->vort, s->stat, s->meanh, s->pull, s->role};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -619,6 +632,47 @@ int wt_sample_raw(wt_solver* s, double* out8) {
 int wt_sync(wt_solver* s) {
   if (!s) return fail(WT_E_ARG, "null argument");
   HIP_TRY(hipStreamSynchronize(s->stream));
+  return WT_OK;
+}
+
+int wt_fence(wt_solver* s, int64_t* ticket) {
+  if (!s || !ticket) return fail(WT_E_ARG, "null argument");
+  const int64_t t = s->fence_seq + 1;
+  const int k = (int)(t % wt_solver::FENCES);
+  if (!s->fence_ev[k]) HIP_TRY(hipEventCreateWithFlags(&s->fence_ev[k], hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(s->fence_ev[k], s->stream));
+  s->fence_seq = t;
+  s->fence_ticket[k] = t;
+  *ticket = t;
+  return WT_OK;
+}
+
+int wt_fence_done(wt_solver* s, int64_t ticket, int32_t* done) {
+  if (!s || !done) return fail(WT_E_ARG, "null argument");
+  if (ticket < 1 || ticket > s->fence_seq) return fail(WT_E_ARG, "wt_fence_done: not a ticket of this handle");
+  *done = 0;
+  if (ticket <= s->fence_done) {
+    *done = 1;
+    return WT_OK;
+  }
+  // the ticket's slot holds it or a newer fence (reused after FENCES newer ones): passing that one
+  // passes this one too; not having passed it yet says nothing, so 0 until it has
+  const int k = (int)(ticket % wt_solver::FENCES);
+  const hipError_t q = hipEventQuery(s->fence_ev[k]);
+  if (q == hipErrorNotReady) return WT_OK;
+  if (q != hipSuccess) return fail(WT_E_HIP, std::string("wt_fence_done: ") + hipGetErrorString(q));
+  s->fence_done = std::max(s->fence_done, s->fence_ticket[k]);
+  *done = 1;
+  return WT_OK;
+}
+
+int wt_fence_wait(wt_solver* s, int64_t ticket) {
+  if (!s) return fail(WT_E_ARG, "null argument");
+  if (ticket < 1 || ticket > s->fence_seq) return fail(WT_E_ARG, "wt_fence_wait: not a ticket of this handle");
+  if (ticket <= s->fence_done) return WT_OK;
+  const int k = (int)(ticket % wt_solver::FENCES);
+  HIP_TRY(hipEventSynchronize(s->fence_ev[k]));
+  s->fence_done = std::max(s->fence_done, s->fence_ticket[k]);
   return WT_OK;
 }
 

@@ -145,6 +145,23 @@ export class SolverGPU {
     this.pollTimer = setTimeout(poll, 0);
   }
 
+  /**
+   * device.queue.onSubmittedWorkDone() for this solver's queue (the handle's stream): a Promise that
+   * resolves once the GPU has finished the work queued on the solver so far. The reference's frame
+   * loop counts frames in flight with it (backend/webgpu.ts:92-94, :120). Polled, never blocks; it
+   * also resolves when the solver is destroyed.
+   */
+  onSubmittedWorkDone() {
+    const t = native.fence(this.h);
+    return new Promise((resolve) => {
+      const poll = () => {
+        if (!this.h || native.fenceDone(this.h, t)) resolve();
+        else setTimeout(poll, 1);
+      };
+      poll();
+    });
+  }
+
   uploadFlags(flags) { native.uploadFlags(this.h, flags); }
   async readFlags() { return native.readFlags(this.h); }
   /** Distances to the wall (cells, < 0 = unknown) for interpolated bounce-back, SolverGPU.ts:312. */

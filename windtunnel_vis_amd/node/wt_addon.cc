@@ -823,6 +823,36 @@ napi_value FlushReads(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// fence(h) -> ticket (a number): the point after all work queued on the handle so far
+napi_value Fence(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  int64_t t = 0;
+  int rc = wt_fence(lk.h->s, &t);
+  if (rc) return throw_wt(env, rc);
+  napi_value out;
+  napi_create_double(env, (double)t, &out);
+  return out;
+}
+
+// fenceDone(h, ticket) -> whether the GPU has passed the fence; never blocks
+napi_value FenceDone(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  HandleLock lk(env, argv[0]);
+  if (!lk) return nullptr;
+  double t = 0;
+  napi_get_value_double(env, argv[1], &t);
+  int32_t done = 0;
+  int rc = wt_fence_done(lk.h->s, (int64_t)t, &done);
+  if (rc) return throw_wt(env, rc);
+  napi_value out;
+  napi_get_boolean(env, done != 0, &out);
+  return out;
+}
+
 napi_value BytesFor(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!args(env, info, 1, argv)) return nullptr;
@@ -847,7 +877,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"derive", Derive}, {"exportFields", ExportFields}, {"probe", Probe}, {"streamlines", Streamlines},
       {"particlesCreate", ParticlesCreate}, {"particlesDestroy", ParticlesDestroy},
       {"particlesUpdate", ParticlesUpdate}, {"particlesRead", ParticlesRead},
-      {"encodePost", EncodePost}, {"flushReads", FlushReads},
+      {"encodePost", EncodePost}, {"flushReads", FlushReads}, {"fence", Fence}, {"fenceDone", FenceDone},
   };
   for (auto& f : fns) {
     napi_value fn;

@@ -250,6 +250,22 @@ class SolverGPU:
     def sync(self):
         N.check(self._lib.wt_sync(self._h))
 
+    def fence(self):
+        """A ticket for the point after all work queued on this solver so far (the queue's
+        onSubmittedWorkDone, backend/webgpu.ts:120, which the reference's frame throttle counts)."""
+        t = N.C.c_int64()
+        N.check(self._lib.wt_fence(self._h, N.C.byref(t)))
+        return t.value
+
+    def fenceDone(self, ticket):
+        """Non-blocking: True once the GPU has passed the fence."""
+        d = N.C.c_int32()
+        N.check(self._lib.wt_fence_done(self._h, int(ticket), N.C.byref(d)))
+        return bool(d.value)
+
+    def fenceWait(self, ticket):
+        N.check(self._lib.wt_fence_wait(self._h, int(ticket)))
+
     # -- readback ---------------------------------------------------------------------------
     def readPopulations(self):
         """Current populations, unpacked to full f_i, shape (q, n) float32."""
