@@ -73,6 +73,8 @@ export declare class SolverGPU implements LbmSolver {
   encodePost(derived?: boolean): void;
   /** Delivers the queued reads to onStats as they complete, keeping `latest` (SolverGPU.ts:264-287). */
   flushReads(onStats?: (s: SolverStats) => void): void;
+  /** device.queue.onSubmittedWorkDone() for this solver's queue (backend/webgpu.ts:120). */
+  onSubmittedWorkDone(): Promise<void>;
   uploadFlags(flags: Uint32Array): void;
   readFlags(): Promise<Uint32Array>;
   uploadSdf(d: Float32Array): void;
