@@ -101,7 +101,8 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const uint32_t* __restrict__
 
 // Groups of CPT cells along x (the bulk kernel's unit) that contain a WALL cell, or mix solid and
 // fluid, go to the wall kernel; their cells are listed with their solid masks.
-// counts[0] = listed cells, counts[1] = bulk fluid cells, counts[2] = bulk solid cells.
+// counts[0] = listed cells, counts[2] = bulk solid cells (every other cell is a bulk fluid cell:
+// the host takes their count as the remainder, which saves an atomic in nearly every wave).
 // The pulled populations of a wall cell as links (see WallLink), following the reference's branch
 // order for a pull (wgslLbm.ts:139-175): inlet, ground (bounce-back + belt), y-z edge mirror,
 // y mirror, z periodic / mirror, solid neighbour (halfway or Bouzidi, with the fallback to q = 1/2
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(BLOCK) void k_collect(const uint8_t* __restrict__ k
                                                    unsigned long long* __restrict__ counts, KParams P) {
   const int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x;  // group index
   const int64_t ngroups = (P.n + CPT - 1) / CPT;
-  int nl = 0, nf = 0, ns = 0;
+  int nl = 0, ns = 0;
   const int64_t c0 = g * CPT;
   if (g < ngroups) {
     bool anyw = false, anys = false, alls = true;
@@ -224,8 +225,7 @@ __global__ __launch_bounds__(BLOCK) void k_collect(const uint8_t* __restrict__ k
     // odd cell counts: a group that runs past the end is always listed
     const bool tail = c0 + CPT > P.n;
     if (anyw || (anys && !alls) || tail) nl = (int)min<int64_t>(CPT, P.n - c0);
-    else if (alls) ns = CPT;
-    else nf = CPT;
+    else if (alls) ns = CPT;  // else: a bulk fluid group (counted by the host as the remainder)
   }
   // wave-aggregated append
   const int lane = threadIdx.x & 63;
@@ -237,12 +237,11 @@ __global__ __launch_bounds__(BLOCK) void k_collect(const uint8_t* __restrict__ k
   }
   const int total = __shfl(incl, 63);
   unsigned long long base = 0;
-  int tf = nf, ts = ns;
+  int ts = ns;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) { tf += __shfl_xor(tf, o); ts += __shfl_xor(ts, o); }
+  for (int o = 32; o > 0; o >>= 1) ts += __shfl_xor(ts, o);
   if (lane == 0) {
     if (total) base = atomicAdd(&counts[0], (unsigned long long)total);
-    if (tf) atomicAdd(&counts[1], (unsigned long long)tf);
     if (ts) atomicAdd(&counts[2], (unsigned long long)ts);
   }
   base = __shfl(base, 0);

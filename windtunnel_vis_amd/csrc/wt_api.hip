@@ -116,7 +116,7 @@ int reclassify(wt_solver* s) {
   HIP_TRY(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->nwall = (int64_t)h[0];
-  s->nfast = (int64_t)h[1];
+  s->nfast = s->n - (int64_t)h[0] - (int64_t)h[2];  // every cell is listed, bulk solid or bulk fluid
   s->nsolid = (int64_t)h[2];
   // the wall list in cell order (the collection appends wave by wave in any order), so that each
   // group of wall blocks covers a short range of rows and can be placed beside them (build_roles)
